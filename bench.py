@@ -1,0 +1,217 @@
+"""Benchmark: n = 54 spin-wave generalized eigensolves per second on one MI355X.
+
+Workload (one "step"): solve every pair of a 65,536-point spin-wave batch,
+n = 54 (27 sublattices), complex128, q on a 64 x 32 x 32 grid in [0,1)^3,
+pair (A, B) = (eta, H(q)) with lambda_min(H) = 10^U(-3,-1) -- generated on the
+device by this library (synthetic data).  Inputs are resident in HBM before
+the timed region; each step is one launch of the drop-in entry point
+geneig_full_n54_batched_launch over the whole batch.
+
+The JSON line carries the roofline of the solve kernel (algorithmic FP64 work
+F(n) = 32 n^3 flop per pair, LAPACK zhegv-equivalent, over the kernel's
+HIP-event duration on its own stream, against the 78.6 TFLOP/s FP64 peak) and
+a CPU baseline: the oracle (a port of the pipeline, OpenMP over pairs on the
+host cores) on a bounded sample, with LAPACK zhegv beside it.
+
+Usage: python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--batch 65536]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+FP64_PEAK = 78.6e12      # MI355X dense FP64 (vector and matrix), FLOP/s
+HBM_PEAK = 8.0e12        # MI355X HBM3E, B/s
+
+
+def flops_per_pair(n):
+    return 32.0 * n ** 3
+
+
+def bytes_per_pair(n):
+    return 48.0 * n * n + 8.0 * n + 4.0
+
+
+def load_package():
+    import importlib.util
+
+    pkg = os.path.join(ROOT, "sunny-batched-geneig_amd")
+    name = "sunny_batched_geneig_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(pkg, "__init__.py"),
+                                                  submodule_search_locations=[pkg])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(A, B, target_s=12.0):
+    """Oracle (port) and LAPACK zhegv, OpenMP over pairs on every host core,
+    on a bounded sample of the same inputs.  Returns (port, lapack) dicts."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    n = A.shape[-1]
+    out = {}
+    for kind, fn in (("port", O.zhegv), ("lapack", O.lapack_zhegv)):
+        try:
+            if kind == "lapack" and O.lapack_load() is None:
+                continue
+            k = min(A.shape[0], max(cores, 64))
+            t = time.perf_counter()
+            fn(A[:k], B[:k], nthreads=cores)
+            dt = time.perf_counter() - t
+            m = int(min(A.shape[0], max(k, target_s * k / max(dt, 1e-6))))
+            t = time.perf_counter()
+            fn(A[:m], B[:m], nthreads=cores)
+            dt = time.perf_counter() - t
+            out[kind] = {"value": m / dt, "unit": "solves/s", "cores": cores,
+                         "sample": "%d of the workload's n=%d pairs, %.1f s wall" % (m, n, dt)}
+            if kind == "lapack":
+                out[kind]["library"] = O.lib().go_lapack_name().decode()
+        except Exception as e:  # a baseline must never sink the GPU measurement
+            log("cpu baseline %s failed: %s" % (kind, e))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--check", type=int, default=256, help="pairs checked against the oracle")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: pairs are independent, a multi-GPU run would "
+              "split the batch into per-GPU ranges with no exchange (see DESIGN.md); "
+              "run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    geneig = load_package()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, nsub, batch = 54, 27, args.batch
+    grid = (64, 32, 32)
+
+    log("generating %d spin-wave pairs (n=%d) on the device" % (batch, n))
+    t0 = time.perf_counter()
+    A, B = geneig.gen_spinwave(nsub, batch, seed=args.seed, grid=grid)
+    W, U, info = geneig.empty_outputs(n, batch, dev)
+    torch.cuda.synchronize()
+    log("generated in %.1f s" % (time.perf_counter() - t0))
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        geneig.full_n54_batched(A, B, W, U, info, stream=stream)
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    # Timed region: K launches, synchronize on both sides.  Events are
+    # recorded on the stream the kernel is launched on.
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        geneig.full_n54_batched(A, B, W, U, info, stream=stream)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    launch_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    ms_per_step = wall * 1e3 / args.steps
+    value = batch / (wall / args.steps)
+    avg_launch_s = float(np.mean(launch_ms)) * 1e-3
+    log("steps done: %.3f ms/step, kernel avg %.3f ms" % (ms_per_step, avg_launch_s * 1e3))
+
+    bad = int((info != 0).sum().item())
+    # accuracy on a spread sample, against the oracle on the same inputs
+    idx = np.linspace(0, batch - 1, num=min(args.check, batch)).astype(np.int64)
+    it = torch.from_numpy(idx).to(dev)
+    Ah = A.index_select(0, it).cpu().numpy()
+    Bh = B.index_select(0, it).cpu().numpy()
+    Wh = W.index_select(0, it).cpu().numpy()
+    Uh = U.index_select(0, it).cpu().numpy()
+    acc = {}
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        Wo, Xo, io = O.zhegv(Ah, Bh)
+        m = O.compare(Ah, Bh, Wh, Uh, Wo, Xo)
+        acc = {"pairs_checked": int(len(idx)), "max_eig_rel_err_scaled": m["eig_rel_scaled"],
+               "max_eig_rel_err": m["eig_rel"], "max_resid_over_n_eps": m["resid"] / (n * 2.220446049250313e-16),
+               "max_orth_over_n_eps": m["orth"] / (n * 2.220446049250313e-16), "info_nonzero": bad}
+    except Exception as e:
+        log("accuracy check failed to run: %s" % e)
+
+    achieved = flops_per_pair(n) * batch / avg_launch_s
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_geneig_full_n54_spinwave.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                t = json.load(f)
+            if t.get("batch") == batch:
+                traffic = t.get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK, "traffic": traffic,
+                "kernel": "geneig_batched_kernel<MODE_FULL,1> (n=54)",
+                "flop_per_pair": flops_per_pair(n), "pairs_per_launch": batch,
+                "avg_launch_ms": avg_launch_s * 1e3,
+                "hbm_achieved_GBps": bytes_per_pair(n) * batch / avg_launch_s / 1e9,
+                "hbm_frac": bytes_per_pair(n) * batch / avg_launch_s / HBM_PEAK}
+
+    cpu = {}
+    if not args.no_cpu:
+        log("cpu baseline (bounded sample)")
+        cb = min(batch, 16384)
+        cpu = cpu_baseline(A[:cb].cpu().numpy(), B[:cb].cpu().numpy(), target_s=args.cpu_seconds)
+
+    line = {
+        "metric": "n=54 geneig solves/s, chosen vs plain FP64 form, max rel eig err <=1e-12",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (device-generated spin-wave pairs, seed %d)" % args.seed,
+        "config": {"workload": "spin-wave (eta, H(q)) pairs, n=54 (27 sublattices), %d q on a "
+                               "64x32x32 grid, complex128" % batch,
+                   "form": "plain FP64 (drop-in geneig_full_n54_batched_launch)",
+                   "global_batch": batch, "n": n, "parallelism": "none (1 GPU)"},
+        "accuracy": acc,
+        "roofline": roofline,
+        "cpu_baseline": cpu.get("port"),
+        "cpu_lapack_zhegv": cpu.get("lapack"),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,91 @@
+/*
+ * geneig.h -- C ABI of the MI355X (gfx950) batched complex Hermitian-definite
+ * generalized eigensolver  A x = lambda B x  (B Hermitian positive definite).
+ *
+ * Data contract (identical to the reference, src/nvrtc/kernel_source.hpp:200-207):
+ *   A (H), B (S)  batch x n x n complex128, column-major, ld = n, pair stride n^2,
+ *                 stored as full Hermitian matrices (the lower triangle of B and
+ *                 the full A are read).
+ *   W             batch x n float64, ascending eigenvalues, stride n.
+ *   U (X)         batch x n x n complex128, B-orthonormal eigenvectors (U^H B U = I),
+ *                 arbitrary column phase.  U must not alias A or B.
+ *   info          batch int32: 0 ok; n + j when the leading minor of order j of B
+ *                 is not positive definite (W, U of that pair are NaN); k > 0
+ *                 when k off-diagonals of the tridiagonal did not converge
+ *                 (LAPACK zhegv numbering).  Written for every pair, so the
+ *                 caller no longer has to zero it.
+ * Every entry point that launches work returns a hipError_t value (0 = success).
+ * batch <= 0 is a no-op returning 0.  Device pointers only; the library
+ * allocates nothing per launch.
+ */
+#ifndef GENEIG_H
+#define GENEIG_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================== */
+/* Static n = 54 entry points.  Replaces src/geneig_fixed.cu.               */
+/* ======================================================================== */
+
+/* Single-pair Cholesky B = L L^H, L lower with zeroed strict upper triangle.
+ * info uses zpotrf numbering (j when minor j is not PD).
+ * Replaces chol_n54_launch, src/geneig_fixed.cu:134-147. */
+int chol_n54_launch(const hipDoubleComplex* d_S, hipDoubleComplex* d_L, int* d_info,
+                    hipStream_t stream);
+unsigned int chol_n54_block_dim_x(void);        /* geneig_fixed.cu:150 */
+unsigned int chol_n54_shared_mem_bytes(void);   /* geneig_fixed.cu:151 */
+
+/* Single-pair reduction M = L^{-1} H L^{-H}; M lower, strict upper zeroed.
+ * Replaces geneig_reduce_n54_launch, src/geneig_fixed.cu:208-222. */
+int geneig_reduce_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
+                             hipDoubleComplex* d_M, int* d_info, hipStream_t stream);
+unsigned int geneig_reduce_n54_block_dim_x(void);       /* geneig_fixed.cu:225 */
+unsigned int geneig_reduce_n54_shared_mem_bytes(void);  /* geneig_fixed.cu:226 */
+
+/* Single-pair reduction + Hermitian eigendecomposition: W and the
+ * eigenvectors V of M (not back-transformed).
+ * Replaces geneig_reduce_eig_n54_launch, src/geneig_fixed.cu:302-317. */
+int geneig_reduce_eig_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
+                                 double* d_W, hipDoubleComplex* d_V, int* d_info,
+                                 hipStream_t stream);
+unsigned int geneig_reduce_eig_n54_block_dim_x(void);       /* geneig_fixed.cu:320 */
+unsigned int geneig_reduce_eig_n54_shared_mem_bytes(void);  /* geneig_fixed.cu:321 */
+unsigned int geneig_reduce_eig_n54_chol_smem_bytes(void);   /* geneig_fixed.cu:322 */
+unsigned int geneig_reduce_eig_n54_heev_smem_bytes(void);   /* geneig_fixed.cu:323 */
+unsigned int geneig_reduce_eig_n54_heev_workspace(void);    /* geneig_fixed.cu:324 */
+
+/* Single-pair full generalized solve.
+ * Replaces geneig_full_n54_launch, src/geneig_fixed.cu:385-400. */
+int geneig_full_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
+                           double* d_W, hipDoubleComplex* d_U, int* d_info,
+                           hipStream_t stream);
+unsigned int geneig_full_n54_block_dim_x(void);       /* geneig_fixed.cu:403 */
+unsigned int geneig_full_n54_shared_mem_bytes(void);  /* geneig_fixed.cu:404 */
+
+/* THE DROP-IN: batched full generalized solve over strided pairs.
+ * Replaces geneig_full_n54_batched_launch, src/geneig_fixed.cu:474-492. */
+int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
+                                   double* d_W, hipDoubleComplex* d_U, int* d_info,
+                                   int batch_size, hipStream_t stream);
+unsigned int geneig_full_n54_batched_block_dim_x(void);       /* geneig_fixed.cu:495 */
+unsigned int geneig_full_n54_batched_shared_mem_bytes(void);  /* geneig_fixed.cu:496 */
+
+/* Batched finish from a given factor L (lower, zero upper) and reduced M:
+ * heev(M) then U = L^{-H} V.
+ * Replaces geneig_finish_n54_launch, src/geneig_fixed.cu:569-587. */
+int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex* d_M,
+                             double* d_W, hipDoubleComplex* d_U, int* d_info,
+                             int batch_size, hipStream_t stream);
+unsigned int geneig_finish_n54_block_dim_x(void);       /* geneig_fixed.cu:590 */
+unsigned int geneig_finish_n54_shared_mem_bytes(void);  /* geneig_fixed.cu:591 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENEIG_H */

@@ -1,0 +1,94 @@
+/*
+ * geneig_solver.h -- C ABI of the runtime-n solver (per-n specialization by
+ * hipRTC, module cache, autotuner).  Replaces the reference's
+ * NvrtcGeneigSolver (src/nvrtc/nvrtc_solver.hpp:81-217).
+ *
+ * Errors: every function returning int returns 0 on success and a nonzero
+ * GENEIG_E_* code otherwise; geneig_last_error() gives the message of the
+ * last failure on the calling thread (the reference threw std::runtime_error,
+ * src/nvrtc/nvrtc_solver.cpp:49-91).
+ */
+#ifndef GENEIG_SOLVER_H
+#define GENEIG_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GENEIG_E_OK 0
+#define GENEIG_E_ARG 1       /* invalid argument */
+#define GENEIG_E_HIP 2       /* HIP runtime / module error */
+#define GENEIG_E_RTC 3       /* hipRTC compile error */
+#define GENEIG_E_FIT 4       /* configuration does not fit the device */
+#define GENEIG_E_TUNE 5      /* autotune found no valid candidate */
+
+/* TuningMode (nvrtc_solver.hpp:38): how the launch shape is chosen. */
+#define GENEIG_TUNING_SUGGESTED 0
+#define GENEIG_TUNING_AUTOTUNE 1
+/* CacheMode (nvrtc_solver.hpp:47): module cache layers. */
+#define GENEIG_CACHE_AUTO 0
+#define GENEIG_CACHE_NODISK 1
+#define GENEIG_CACHE_NOCACHE 2
+
+typedef struct geneig_solver* geneig_solver_t;
+
+/* Constructor, NvrtcGeneigSolver(n, device_id, bpb, force_block_dim_x,
+ * TuningMode, CacheMode) (nvrtc_solver.hpp:99-104, nvrtc_solver.cpp:927-1047).
+ * bpb_request <= 0 selects pairs-per-workgroup automatically; force_block_dim_x
+ * > 0 fixes threads per pair (64 = one wavefront, the only shape built) and
+ * skips autotuning. */
+int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_request,
+                         int force_block_dim_x, int tuning_mode, int cache_mode);
+void geneig_solver_destroy(geneig_solver_t s);
+
+/* NvrtcGeneigSolver::launch (nvrtc_solver.cpp:1063-1076).  Unlike the
+ * reference, a partial last workgroup is handled (no padding needed). */
+int geneig_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W,
+                         void* d_U, int* d_info, int batch_size, hipStream_t stream);
+/* NvrtcGeneigSolver::launch_chunked (nvrtc_solver.cpp:1078-1107). */
+int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void* d_S,
+                                 double* d_W, void* d_U, int* d_info, int total_matrices,
+                                 int chunk_size, int num_streams, const hipStream_t* streams);
+
+/* Accessors (nvrtc_solver.hpp:120-140). */
+int geneig_solver_matrix_size(geneig_solver_t s);
+int geneig_solver_batches_per_block(geneig_solver_t s);
+void geneig_solver_block_dim(geneig_solver_t s, unsigned* x, unsigned* y, unsigned* z);
+unsigned geneig_solver_shared_mem_bytes(geneig_solver_t s);
+int geneig_solver_autotuned_block_dim_x(geneig_solver_t s);
+int geneig_solver_was_autotuned(geneig_solver_t s);
+int geneig_solver_autotune_sweep_ran(geneig_solver_t s);
+const char* geneig_solver_cache_layer_used(geneig_solver_t s); /* "in-process"|"disk"|"compile" */
+double geneig_solver_acquire_module_ms(geneig_solver_t s);
+/* Eigen-algorithm variant of the loaded module (0 = tridiagonal QL). */
+int geneig_solver_variant(geneig_solver_t s);
+
+/* NvrtcGeneigSolver::cache_dir / cubin_path (nvrtc_solver.cpp:403-431).
+ * Copies a NUL-terminated path into buf; returns the full length. */
+size_t geneig_cache_dir(char* buf, size_t len);
+size_t geneig_cache_path(int n, int bpb, const char* arch, int block_dim_x, int variant,
+                         char* buf, size_t len);
+
+/* Device-free hipRTC compile of the per-n kernel into a code object (ELF).
+ * Used by the build check and by the cache tests; *size receives the object
+ * size, and when buf != NULL and cap >= size the object is copied there. */
+int geneig_rtc_compile(int n, int bpb, int variant, const char* arch, void* buf, size_t cap,
+                       size_t* size);
+uint64_t geneig_source_hash(void);
+uint64_t geneig_fnv1a64(const void* data, size_t len, uint64_t seed);
+
+/* Helpers for launch_chunked callers (make_streams/destroy_streams,
+ * nvrtc_solver.cpp:1109-1125). */
+int geneig_make_streams(int n, hipStream_t* out);
+void geneig_destroy_streams(int n, hipStream_t* streams);
+
+const char* geneig_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENEIG_SOLVER_H */

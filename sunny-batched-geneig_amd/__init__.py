@@ -1,0 +1,346 @@
+"""Python host binding of libgeneig.so (the MI355X batched generalized eigensolver).
+
+Mirrors the reference's host interface for the solve path:
+  * the static n = 54 extern "C" launchers of src/geneig_fixed.cu
+    (``full_n54_batched``, ``chol_n54``, ``reduce_n54``, ``reduce_eig_n54``,
+    ``full_n54``, ``finish_n54``), and
+  * ``GeneigSolver``, the runtime-n solver with the constructor arguments,
+    ``launch``/``launch_chunked`` and introspection of
+    ``NvrtcGeneigSolver`` (src/nvrtc/nvrtc_solver.hpp:81-217).
+Device buffers are torch tensors (complex128 / float64 / int32 on a ROCm
+device); they are plumbing only -- every solve runs the library's HIP kernels
+through the C ABI, and a missing or unloadable library raises at once.
+Errors surface as ``GeneigError`` carrying the library's message (the
+reference threw std::runtime_error).
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgeneig.so")
+
+TUNING_SUGGESTED, TUNING_AUTOTUNE = 0, 1
+CACHE_AUTO, CACHE_NODISK, CACHE_NOCACHE = 0, 1, 2
+
+
+class GeneigError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def build(jobs=4):
+    """Compile libgeneig.so in-tree (hipcc, gfx950)."""
+    import subprocess
+
+    subprocess.check_call(["make", "-s", "-C", HERE, "-j%d" % jobs])
+
+
+def lib():
+    """Load libgeneig.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GeneigError("libgeneig.so is not built (run __graft_entry__.build() or make -C %s)" % HERE)
+    L = C.CDLL(LIB_PATH)
+    P, I, U64, D = C.c_void_p, C.c_int, C.c_uint64, C.c_double
+    sig = {
+        "chol_n54_launch": (I, [P, P, P, P]),
+        "geneig_reduce_n54_launch": (I, [P, P, P, P, P]),
+        "geneig_reduce_eig_n54_launch": (I, [P, P, P, P, P, P]),
+        "geneig_full_n54_launch": (I, [P, P, P, P, P, P]),
+        "geneig_full_n54_batched_launch": (I, [P, P, P, P, P, I, P]),
+        "geneig_finish_n54_launch": (I, [P, P, P, P, P, I, P]),
+        "geneig_solver_create": (I, [P, I, I, I, I, I, I]),
+        "geneig_solver_destroy": (None, [P]),
+        "geneig_solver_launch": (I, [P, P, P, P, P, P, I, P]),
+        "geneig_solver_launch_chunked": (I, [P, P, P, P, P, P, I, I, I, P]),
+        "geneig_solver_matrix_size": (I, [P]),
+        "geneig_solver_batches_per_block": (I, [P]),
+        "geneig_solver_block_dim": (None, [P, P, P, P]),
+        "geneig_solver_shared_mem_bytes": (C.c_uint, [P]),
+        "geneig_solver_autotuned_block_dim_x": (I, [P]),
+        "geneig_solver_was_autotuned": (I, [P]),
+        "geneig_solver_autotune_sweep_ran": (I, [P]),
+        "geneig_solver_cache_layer_used": (C.c_char_p, [P]),
+        "geneig_solver_acquire_module_ms": (D, [P]),
+        "geneig_solver_variant": (I, [P]),
+        "geneig_cache_dir": (C.c_size_t, [C.c_char_p, C.c_size_t]),
+        "geneig_cache_path": (C.c_size_t, [I, I, C.c_char_p, I, I, C.c_char_p, C.c_size_t]),
+        "geneig_rtc_compile": (I, [I, I, I, C.c_char_p, P, C.c_size_t, P]),
+        "geneig_source_hash": (U64, []),
+        "geneig_fnv1a64": (U64, [P, C.c_size_t, U64]),
+        "geneig_make_streams": (I, [I, P]),
+        "geneig_destroy_streams": (None, [I, P]),
+        "geneig_last_error": (C.c_char_p, []),
+        "geneig_gen_generic_launch": (I, [I, I, U64, U64, P, P, P]),
+        "geneig_gen_spinwave_h0_launch": (I, [I, I, U64, I, I, I, U64, I, D, P, P, P]),
+        "geneig_spinwave_shift_launch": (I, [I, I, U64, U64, P, P, P]),
+        "geneig_fill_identity_launch": (I, [I, I, P, P]),
+    }
+    for name in ("chol_n54", "geneig_reduce_n54", "geneig_reduce_eig_n54", "geneig_full_n54",
+                 "geneig_full_n54_batched", "geneig_finish_n54"):
+        sig[name + "_block_dim_x"] = (C.c_uint, [])
+        sig[name + "_shared_mem_bytes"] = (C.c_uint, [])
+    for extra in ("geneig_reduce_eig_n54_chol_smem_bytes", "geneig_reduce_eig_n54_heev_smem_bytes",
+                  "geneig_reduce_eig_n54_heev_workspace"):
+        sig[extra] = (C.c_uint, [])
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().geneig_last_error().decode(errors="replace")
+
+
+def _check_hip(rc, what):
+    if rc != 0:
+        raise GeneigError("%s failed: hipError %d" % (what, rc))
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise GeneigError("%s failed (code %d): %s" % (what, rc, last_error()))
+
+
+# ---- torch plumbing ------------------------------------------------------------
+def _torch():
+    import torch
+
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _require(t, dtype, name):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise GeneigError("%s must be a device tensor" % name)
+    if t.dtype != dtype:
+        raise GeneigError("%s must be %s, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise GeneigError("%s must be contiguous" % name)
+
+
+def _check_batch(n, batch, A, B, W, U, info):
+    torch = _torch()
+    for t, nm in ((A, "A"), (B, "B"), (U, "U")):
+        _require(t, torch.complex128, nm)
+        if t.numel() < batch * n * n:
+            raise GeneigError("%s holds %d elements, batch*n*n = %d" % (nm, t.numel(), batch * n * n))
+    if W is not None:
+        _require(W, torch.float64, "W")
+        if W.numel() < batch * n:
+            raise GeneigError("W holds %d elements, batch*n = %d" % (W.numel(), batch * n))
+    _require(info, torch.int32, "info")
+    if info.numel() < batch:
+        raise GeneigError("info holds %d elements, batch = %d" % (info.numel(), batch))
+
+
+def empty_outputs(n, batch, device="cuda"):
+    torch = _torch()
+    W = torch.empty((batch, n), dtype=torch.float64, device=device)
+    U = torch.empty((batch, n, n), dtype=torch.complex128, device=device)
+    info = torch.empty((batch,), dtype=torch.int32, device=device)
+    return W, U, info
+
+
+# ---- static n = 54 entry points (src/geneig_fixed.cu) ---------------------------
+def full_n54_batched(H, S, W, U, info, batch=None, stream=None):
+    """geneig_full_n54_batched_launch (geneig_fixed.cu:474-492)."""
+    batch = H.shape[0] if batch is None else batch
+    _check_batch(54, batch, H, S, W, U, info)
+    _check_hip(lib().geneig_full_n54_batched_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
+                                                    int(batch), _stream(stream)),
+               "geneig_full_n54_batched_launch")
+
+
+def full_n54(H, S, W, U, info, stream=None):
+    """geneig_full_n54_launch (geneig_fixed.cu:385-400), one pair."""
+    _check_batch(54, 1, H, S, W, U, info)
+    _check_hip(lib().geneig_full_n54_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info), _stream(stream)),
+               "geneig_full_n54_launch")
+
+
+def chol_n54(S, L, info, stream=None):
+    """chol_n54_launch (geneig_fixed.cu:134-147), one pair."""
+    _check_batch(54, 1, S, S, None, L, info)
+    _check_hip(lib().chol_n54_launch(_ptr(S), _ptr(L), _ptr(info), _stream(stream)), "chol_n54_launch")
+
+
+def reduce_n54(H, S, M, info, stream=None):
+    """geneig_reduce_n54_launch (geneig_fixed.cu:208-222), one pair."""
+    _check_batch(54, 1, H, S, None, M, info)
+    _check_hip(lib().geneig_reduce_n54_launch(_ptr(H), _ptr(S), _ptr(M), _ptr(info), _stream(stream)),
+               "geneig_reduce_n54_launch")
+
+
+def reduce_eig_n54(H, S, W, V, info, stream=None):
+    """geneig_reduce_eig_n54_launch (geneig_fixed.cu:302-317), one pair."""
+    _check_batch(54, 1, H, S, W, V, info)
+    _check_hip(lib().geneig_reduce_eig_n54_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(V), _ptr(info),
+                                                  _stream(stream)), "geneig_reduce_eig_n54_launch")
+
+
+def finish_n54(L, M, W, U, info, batch=None, stream=None):
+    """geneig_finish_n54_launch (geneig_fixed.cu:569-587)."""
+    batch = L.shape[0] if batch is None else batch
+    _check_batch(54, batch, L, M, W, U, info)
+    _check_hip(lib().geneig_finish_n54_launch(_ptr(L), _ptr(M), _ptr(W), _ptr(U), _ptr(info), int(batch),
+                                              _stream(stream)), "geneig_finish_n54_launch")
+
+
+# ---- runtime-n solver (src/nvrtc/nvrtc_solver.hpp:81-217) -------------------------
+class GeneigSolver:
+    """Per-n specialized solver: GeneigSolver(n, device_id, bpb, force_block_dim_x, tuning_mode, cache_mode)."""
+
+    def __init__(self, n, device_id=0, batches_per_block_request=0, force_block_dim_x=0,
+                 tuning_mode=TUNING_SUGGESTED, cache_mode=CACHE_AUTO):
+        self._h = C.c_void_p()
+        rc = lib().geneig_solver_create(C.byref(self._h), int(n), int(device_id), int(batches_per_block_request),
+                                        int(force_block_dim_x), int(tuning_mode), int(cache_mode))
+        _check(rc, "GeneigSolver(n=%d)" % n)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.geneig_solver_destroy(h)
+            self._h = None
+
+    def launch(self, H, S, W, U, info, batch=None, stream=None):
+        batch = H.shape[0] if batch is None else batch
+        _check_batch(self.matrix_size(), batch, H, S, W, U, info)
+        _check(lib().geneig_solver_launch(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
+                                          int(batch), _stream(stream)), "GeneigSolver.launch")
+
+    def launch_chunked(self, H, S, W, U, info, total, chunk_size, streams):
+        _check_batch(self.matrix_size(), total, H, S, W, U, info)
+        arr = (C.c_void_p * len(streams))(*[s.cuda_stream for s in streams])
+        _check(lib().geneig_solver_launch_chunked(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
+                                                  int(total), int(chunk_size), len(streams),
+                                                  C.cast(arr, C.c_void_p)), "GeneigSolver.launch_chunked")
+
+    def matrix_size(self):
+        return lib().geneig_solver_matrix_size(self._h)
+
+    def batches_per_block(self):
+        return lib().geneig_solver_batches_per_block(self._h)
+
+    def block_dim(self):
+        x, y, z = C.c_uint(), C.c_uint(), C.c_uint()
+        lib().geneig_solver_block_dim(self._h, C.byref(x), C.byref(y), C.byref(z))
+        return (x.value, y.value, z.value)
+
+    def shared_mem_bytes(self):
+        return lib().geneig_solver_shared_mem_bytes(self._h)
+
+    def autotuned_block_dim_x(self):
+        return lib().geneig_solver_autotuned_block_dim_x(self._h)
+
+    def was_autotuned(self):
+        return bool(lib().geneig_solver_was_autotuned(self._h))
+
+    def autotune_sweep_ran(self):
+        return bool(lib().geneig_solver_autotune_sweep_ran(self._h))
+
+    def cache_layer_used(self):
+        return lib().geneig_solver_cache_layer_used(self._h).decode()
+
+    def acquire_module_ms(self):
+        return lib().geneig_solver_acquire_module_ms(self._h)
+
+    def variant(self):
+        return lib().geneig_solver_variant(self._h)
+
+    @staticmethod
+    def cache_dir():
+        buf = C.create_string_buffer(4096)
+        lib().geneig_cache_dir(buf, 4096)
+        return buf.value.decode()
+
+    @staticmethod
+    def cache_path(n, bpb, arch, block_dim_x=64, variant=0):
+        buf = C.create_string_buffer(4096)
+        lib().geneig_cache_path(int(n), int(bpb), arch.encode(), int(block_dim_x), int(variant), buf, 4096)
+        return buf.value.decode()
+
+
+def rtc_compile(n, bpb=1, variant=0, arch="gfx950"):
+    """Device-free hipRTC compile of the per-n kernel; returns the code object bytes."""
+    size = C.c_size_t(0)
+    _check(lib().geneig_rtc_compile(int(n), int(bpb), int(variant), arch.encode(), None, 0, C.byref(size)),
+           "rtc_compile(n=%d)" % n)
+    buf = C.create_string_buffer(size.value)
+    _check(lib().geneig_rtc_compile(int(n), int(bpb), int(variant), arch.encode(), buf, size.value,
+                                    C.byref(size)), "rtc_compile(n=%d)" % n)
+    return buf.raw[: size.value]
+
+
+def make_streams(n):
+    torch = _torch()
+    return [torch.cuda.Stream() for _ in range(n)]
+
+
+# ---- device generators ------------------------------------------------------------
+def gen_generic(n, batch, seed, first_pair=0, device="cuda", stream=None):
+    """A = (M + M^H)/(2 sqrt n), B = G G^H / n + I on the device."""
+    torch = _torch()
+    A = torch.empty((batch, n, n), dtype=torch.complex128, device=device)
+    B = torch.empty_like(A)
+    _check_hip(lib().geneig_gen_generic_launch(int(n), int(batch), int(seed), int(first_pair), _ptr(A), _ptr(B),
+                                               _stream(stream)), "geneig_gen_generic_launch")
+    return A, B
+
+
+def gen_spinwave(nsub, batch, seed, grid=(64, 32, 32), first_q=0, nbonds=12, jscale=1.0, device="cuda",
+                 chunk=8192, stream=None):
+    """Spin-wave pairs (eta, H(q)) with lambda_min(H) = 10^U(-3,-1) per q.
+
+    lambda_min(H0) is taken from this library's own solve of (H0, I), chunked
+    to bound the scratch memory; the shift kernel then sets lambda_min.
+    """
+    torch = _torch()
+    n = 2 * nsub
+    A = torch.empty((batch, n, n), dtype=torch.complex128, device=device)
+    B = torch.empty_like(A)
+    s = _stream(stream)
+    _check_hip(lib().geneig_gen_spinwave_h0_launch(int(nsub), int(nbonds), int(seed), grid[0], grid[1], grid[2],
+                                                   int(first_q), int(batch), float(jscale), _ptr(A), _ptr(B), s),
+               "geneig_gen_spinwave_h0_launch")
+    solver = None if n == 54 else GeneigSolver(n, device_id=torch.device(device).index or 0)
+    c = min(chunk, batch)
+    Hc = torch.empty((c, n, n), dtype=torch.complex128, device=device)
+    Ic = torch.empty_like(Hc)
+    Wc, Uc, infoc = empty_outputs(n, c, device)
+    _check_hip(lib().geneig_fill_identity_launch(n, c, _ptr(Ic), s), "geneig_fill_identity_launch")
+    worst = torch.zeros((), dtype=torch.int32, device=device)
+    for off in range(0, batch, c):
+        m = min(c, batch - off)
+        Hc[:m].copy_(B[off:off + m])
+        if solver is None:
+            full_n54_batched(Hc, Ic, Wc, Uc, infoc, batch=m, stream=stream)
+        else:
+            solver.launch(Hc, Ic, Wc, Uc, infoc, batch=m, stream=stream)
+        _check_hip(lib().geneig_spinwave_shift_launch(n, m, int(seed), int(first_q) + off, _ptr(Wc),
+                                                      C.c_void_p(B[off].data_ptr()), s),
+                   "geneig_spinwave_shift_launch")
+        worst = torch.maximum(worst, infoc[:m].abs().max())
+    if int(worst.item()) != 0:
+        raise GeneigError("spin-wave generator: lambda_min solve reported info != 0")
+    del Hc, Ic, Wc, Uc, infoc
+    return A, B

@@ -1,0 +1,661 @@
+// geneig_solver.cpp -- runtime-n solver: hipRTC per-n specialization, the
+// three-layer module cache and the autotuner.  MI355X counterpart of the
+// reference's NvrtcGeneigSolver (src/nvrtc/nvrtc_solver.{hpp,cpp}).
+//
+// What is specialized per n: the whole pipeline of geneig_device.inc is
+// compiled by hipRTC with -DGENEIG_RTC_N=n and -DGENEIG_RTC_BPB=pairs per
+// workgroup, so every loop bound and LDS offset is a compile-time constant.
+// A compile takes seconds (no linker step against a closed library), so the
+// disk cache mostly saves start-up time rather than minutes.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <sys/types.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/geneig_solver.h"
+#include "geneig_device_src.inc"  // kGeneigDeviceSource: geneig_device.inc as a string
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    std::fprintf(stderr, "[geneig] %s\n", buf);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess)                                                             \
+            return fail(GENEIG_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));    \
+    } while (0)
+
+// ---- content hashes (FNV-1a 64, the reference's cache fingerprint,
+//      src/nvrtc/nvrtc_solver.cpp:321-333) ------------------------------------
+constexpr uint64_t kFnvOffset = 0xcbf29ce484222325ULL;
+constexpr uint64_t kFnvPrime = 0x100000001b3ULL;
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = kFnvOffset) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= kFnvPrime;
+    }
+    return h;
+}
+
+uint64_t source_hash() {
+    static const uint64_t h = fnv1a(kGeneigDeviceSource, std::strlen(kGeneigDeviceSource));
+    return h;
+}
+
+// Toolchain fingerprint: hipRTC version plus the compile options, so a code
+// object built by another compiler or with other flags is never reused.
+const char* const kRtcFlags[] = {"-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                 "-Xclang", "-packed-fp32-ops"};
+
+uint64_t toolchain_hash() {
+    static const uint64_t h = [] {
+        int major = 0, minor = 0;
+        hiprtcVersion(&major, &minor);
+        uint64_t x = fnv1a(&major, sizeof major);
+        x = fnv1a(&minor, sizeof minor, x);
+        for (const char* f : kRtcFlags) x = fnv1a(f, std::strlen(f), x);
+        return x;
+    }();
+    return h;
+}
+
+std::string sanitize(const std::string& s) {
+    std::string o = s;
+    for (char& c : o)
+        if (c == ':' || c == '/' || c == '+' || c == ' ') c = '_';
+    return o;
+}
+
+std::string cache_dir_str() {
+    if (const char* env = std::getenv("SUNNY_GENEIG_CACHE_DIR")) return env;
+    if (const char* xdg = std::getenv("XDG_CACHE_HOME")) return std::string(xdg) + "/sunny_geneig/hsaco";
+    if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/sunny_geneig/hsaco";
+    return "/tmp/sunny_geneig/hsaco";
+}
+
+std::string cache_path_str(int n, int bpb, const std::string& arch, int bdx, int variant) {
+    char buf[1024];
+    std::snprintf(buf, sizeof buf, "%s/%d_%d_%s_%d_v%d_lib%016llx_src%016llx.hsaco",
+                  cache_dir_str().c_str(), n, bpb, sanitize(arch).c_str(), bdx, variant,
+                  (unsigned long long)toolchain_hash(), (unsigned long long)source_hash());
+    return buf;
+}
+
+std::string tune_path_str(int n, const std::string& arch) {
+    char buf[1024];
+    std::snprintf(buf, sizeof buf, "%s/autotune_%d_%s_lib%016llx_src%016llx.txt",
+                  cache_dir_str().c_str(), n, sanitize(arch).c_str(),
+                  (unsigned long long)toolchain_hash(), (unsigned long long)source_hash());
+    return buf;
+}
+
+void mkdir_p(const std::string& path) {
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') ::mkdir(path.substr(0, i).c_str(), 0755);
+    ::mkdir(path.c_str(), 0755);
+}
+
+// Atomic write: tmp file in the same directory, then rename (multi-process
+// safe, as src/nvrtc/nvrtc_solver.cpp:481-520).
+bool write_atomic(const std::string& path, const void* data, size_t size) {
+    mkdir_p(path.substr(0, path.find_last_of('/')));
+    char tmp[1200];
+    std::snprintf(tmp, sizeof tmp, "%s.tmp.%d.%lld", path.c_str(), (int)::getpid(),
+                  (long long)std::chrono::steady_clock::now().time_since_epoch().count());
+    {
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f) return false;
+        f.write(static_cast<const char*>(data), (std::streamsize)size);
+        if (!f) {
+            f.close();
+            std::remove(tmp);
+            return false;
+        }
+    }
+    if (std::rename(tmp, path.c_str()) != 0) {
+        std::remove(tmp);
+        return false;
+    }
+    return true;
+}
+
+// ---- hipRTC ----------------------------------------------------------------
+int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vector<char>& code) {
+    if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
+    if (bpb < 1 || bpb > 16) return fail(GENEIG_E_ARG, "pairs per workgroup %d outside 1..16", bpb);
+    if (variant != 0) return fail(GENEIG_E_ARG, "eigen-algorithm variant %d is not built", variant);
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, kGeneigDeviceSource, "geneig_device.hip", 0, nullptr, nullptr) !=
+        HIPRTC_SUCCESS)
+        return fail(GENEIG_E_RTC, "hiprtcCreateProgram failed");
+    std::vector<std::string> opts = {"--offload-arch=" + arch, "-DGENEIG_RTC_N=" + std::to_string(n),
+                                     "-DGENEIG_RTC_BPB=" + std::to_string(bpb),
+                                     "-DGENEIG_RTC_VARIANT=" + std::to_string(variant)};
+    for (const char* f : kRtcFlags) opts.push_back(f);
+    std::vector<const char*> p;
+    for (auto& s : opts) p.push_back(s.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)p.size(), p.data());
+    if (r != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return fail(GENEIG_E_RTC, "hipRTC compile failed (n=%d bpb=%d variant=%d arch=%s): %s\n%s", n,
+                    bpb, variant, arch.c_str(), hiprtcGetErrorString(r), log.c_str());
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return GENEIG_E_OK;
+}
+
+// ---- loaded module -----------------------------------------------------------
+struct Module {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    unsigned lds_per_pair = 0;
+    int n = 0, bpb = 0, variant = 0;
+    Module() = default;
+    Module(const Module&) = delete;
+    Module& operator=(const Module&) = delete;
+    ~Module() {
+        if (mod) (void)hipModuleUnload(mod);
+    }
+};
+
+int load_module(const std::vector<char>& code, int n, int bpb, int variant,
+                std::shared_ptr<Module>& out) {
+    auto m = std::make_shared<Module>();
+    m->n = n;
+    m->bpb = bpb;
+    m->variant = variant;
+    HIP_TRY(hipModuleLoadData(&m->mod, code.data()));
+    HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_full_kernel"));
+    hipDeviceptr_t ptr = nullptr;
+    size_t size = 0;
+    HIP_TRY(hipModuleGetGlobal(&ptr, &size, m->mod, "geneig_lds_bytes_per_pair"));
+    if (size != sizeof(unsigned)) return fail(GENEIG_E_HIP, "geneig_lds_bytes_per_pair has size %zu", size);
+    HIP_TRY(hipMemcpyDtoH(&m->lds_per_pair, ptr, size));
+    out = m;
+    return GENEIG_E_OK;
+}
+
+struct Key {
+    int n, bpb, variant, bdx;
+    std::string arch;
+    bool operator<(const Key& o) const {
+        return std::tie(n, bpb, variant, bdx, arch) < std::tie(o.n, o.bpb, o.variant, o.bdx, o.arch);
+    }
+};
+
+std::mutex g_module_mu;
+std::map<Key, std::weak_ptr<Module>> g_modules;
+
+struct TuneWinner {
+    int bpb = 0, variant = 0;
+    double pairs_per_s = 0.0;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<int, std::string>, TuneWinner> g_tune;
+
+// Synthetic tuning pairs (the reference's generators, nvrtc_solver.cpp:606-638,
+// restated with a counter-based RNG): A = M + M^H, B = G G^H + n I.
+uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double gauss(uint64_t key) {
+    double u1 = ((double)(mix64(key) >> 11) + 0.5) * 0x1.0p-53;
+    double u2 = ((double)(mix64(key ^ 0xA5A5A5A5A5A5A5A5ULL) >> 11) + 0.5) * 0x1.0p-53;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+void make_tune_pairs(int n, int batch, std::vector<std::complex<double>>& A,
+                     std::vector<std::complex<double>>& B) {
+    const size_t nn = (size_t)n * n;
+    A.assign(nn * batch, 0.0);
+    B.assign(nn * batch, 0.0);
+    std::vector<std::complex<double>> M(nn), G(nn);
+    for (int b = 0; b < batch; ++b) {
+        for (size_t t = 0; t < nn; ++t) {
+            uint64_t k = ((uint64_t)n << 48) ^ ((uint64_t)b << 22) ^ (t << 1);
+            M[t] = {gauss(4 * k), gauss(4 * k + 1)};
+            G[t] = {gauss(4 * k + 2), gauss(4 * k + 3)};
+        }
+        auto* Ab = &A[nn * b];
+        auto* Bb = &B[nn * b];
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) Ab[i + (size_t)j * n] = M[i + (size_t)j * n] + std::conj(M[j + (size_t)i * n]);
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < n; ++k) {
+                const std::complex<double> gjk = std::conj(G[j + (size_t)k * n]);
+                for (int i = 0; i < n; ++i) Bb[i + (size_t)j * n] += G[i + (size_t)k * n] * gjk;
+            }
+        for (int i = 0; i < n; ++i) Bb[i + (size_t)i * n] += (double)n;
+    }
+}
+
+double median(std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    return v[v.size() / 2];
+}
+
+}  // namespace
+
+// ---- the solver object ---------------------------------------------------------
+struct geneig_solver {
+    int n = 0, device = 0, bpb = 0, variant = 0;
+    std::string arch;
+    size_t lds_per_cu = 0, lds_per_block = 0;
+    std::shared_ptr<Module> module;
+    unsigned shared_bytes = 0;
+    int autotuned_bdx = 0;
+    bool was_autotuned = false, sweep_ran = false;
+    const char* layer = "compile";
+    double acquire_ms = 0.0;
+
+    // Module acquisition through the cache layers (nvrtc_solver.cpp:522-597).
+    int acquire(int want_bpb, int want_variant, int cache_mode, std::shared_ptr<Module>& out,
+                const char** layer_out) {
+        const Key key{n, want_bpb, want_variant, 64, arch};
+        if (cache_mode != GENEIG_CACHE_NOCACHE) {
+            std::lock_guard<std::mutex> lk(g_module_mu);
+            auto it = g_modules.find(key);
+            if (it != g_modules.end()) {
+                if (auto h = it->second.lock()) {
+                    out = h;
+                    *layer_out = "in-process";
+                    return GENEIG_E_OK;
+                }
+                g_modules.erase(it);
+            }
+        }
+        const std::string path = cache_path_str(n, want_bpb, arch, 64, want_variant);
+        if (cache_mode == GENEIG_CACHE_AUTO) {
+            std::ifstream f(path, std::ios::binary | std::ios::ate);
+            if (f) {
+                const std::streamsize sz = f.tellg();
+                f.seekg(0);
+                std::vector<char> code((size_t)std::max<std::streamsize>(sz, 0));
+                bool ok = sz >= 1024 && (bool)f.read(code.data(), sz);
+                // Code objects are ELF: 0x7F 'E' 'L' 'F' (nvrtc_solver.cpp:466-474).
+                ok = ok && (unsigned char)code[0] == 0x7F && code[1] == 'E' && code[2] == 'L' &&
+                     code[3] == 'F';
+                f.close();
+                std::shared_ptr<Module> h;
+                if (ok && load_module(code, n, want_bpb, want_variant, h) == GENEIG_E_OK) {
+                    std::lock_guard<std::mutex> lk(g_module_mu);
+                    g_modules[key] = h;
+                    out = h;
+                    *layer_out = "disk";
+                    return GENEIG_E_OK;
+                }
+                std::fprintf(stderr, "[cache] invalid code object, removing: %s\n", path.c_str());
+                std::remove(path.c_str());
+            }
+        }
+        std::vector<char> code;
+        int rc = rtc_compile(n, want_bpb, want_variant, arch, code);
+        if (rc) return rc;
+        std::shared_ptr<Module> h;
+        rc = load_module(code, n, want_bpb, want_variant, h);
+        if (rc) return rc;
+        if (cache_mode == GENEIG_CACHE_AUTO && !write_atomic(path, code.data(), code.size()))
+            std::fprintf(stderr, "[cache] could not write %s\n", path.c_str());
+        if (cache_mode != GENEIG_CACHE_NOCACHE) {
+            std::lock_guard<std::mutex> lk(g_module_mu);
+            g_modules[key] = h;
+        }
+        out = h;
+        *layer_out = "compile";
+        return GENEIG_E_OK;
+    }
+
+    bool fits(int b, unsigned per_pair) const { return (size_t)b * per_pair <= lds_per_block; }
+
+    // Pairs per workgroup that maximizes resident pairs per CU (LDS-limited);
+    // ties go to the larger workgroup (fewer workgroups to dispatch).
+    int suggest_bpb(unsigned per_pair) const {
+        int best = 0;
+        size_t best_res = 0;
+        for (int b : {16, 8, 4, 2, 1}) {
+            if (!fits(b, per_pair)) continue;
+            size_t res = (size_t)b * (lds_per_cu / ((size_t)b * per_pair));
+            if (res > best_res) {
+                best_res = res;
+                best = b;
+            }
+        }
+        return best;
+    }
+
+    int launch_raw(hipFunction_t fn, int b, unsigned bytes, const void* H, const void* S, double* W,
+                   void* U, int* info, int batch, hipStream_t stream) {
+        if (batch <= 0) return GENEIG_E_OK;
+        const void* h = H;
+        const void* s = S;
+        void* args[] = {&h, &s, &W, &U, &info, &batch};
+        const unsigned grid = (unsigned)((batch + b - 1) / b);
+        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, 64, (unsigned)b, 1, bytes, stream, args, nullptr));
+        return GENEIG_E_OK;
+    }
+
+    int autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands, TuneWinner& win);
+};
+
+int geneig_solver::autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands,
+                            TuneWinner& win) {
+    // Workload and validation follow src/nvrtc/nvrtc_solver.cpp:649-925: 1024
+    // pairs, 5-sample median timing, 8 spot pairs checked against the first
+    // candidate at 1e-10 (eigenvalues scaled by max(|w|,1), and 1 - |u^H B u_ref|).
+    const int T = 1024, SPOT = 8;
+    const double tol = 1e-10;
+    std::vector<std::complex<double>> A, B;
+    make_tune_pairs(n, T, A, B);
+    const size_t nn = (size_t)n * n;
+    void *dA = nullptr, *dB = nullptr, *dU = nullptr;
+    double* dW = nullptr;
+    int* dI = nullptr;
+    auto cleanup = [&] {
+        (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dU); (void)hipFree(dW); (void)hipFree(dI);
+    };
+    if (hipMalloc(&dA, nn * T * 16) != hipSuccess || hipMalloc(&dB, nn * T * 16) != hipSuccess ||
+        hipMalloc(&dU, nn * T * 16) != hipSuccess || hipMalloc((void**)&dW, (size_t)n * T * 8) != hipSuccess ||
+        hipMalloc((void**)&dI, (size_t)T * 4) != hipSuccess) {
+        cleanup();
+        return fail(GENEIG_E_HIP, "autotune: device allocation failed");
+    }
+    (void)hipMemcpy(dA, A.data(), nn * T * 16, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dB, B.data(), nn * T * 16, hipMemcpyHostToDevice);
+    std::vector<double> Wref, Wc((size_t)n * T);
+    std::vector<std::complex<double>> Uref, Uc(nn * T);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    bool have = false;
+    for (size_t ci = 0; ci < cands.size(); ++ci) {
+        const int b = cands[ci].first, v = cands[ci].second;
+        std::shared_ptr<Module> m;
+        const char* lay = nullptr;
+        if (acquire(b, v, cache_mode, m, &lay) != GENEIG_E_OK) {
+            std::fprintf(stderr, "[autotune] bpb=%d variant=%d: rejected at compile/load\n", b, v);
+            continue;
+        }
+        const unsigned bytes = m->lds_per_pair * (unsigned)b;
+        if (!fits(b, m->lds_per_pair)) {
+            std::fprintf(stderr, "[autotune] bpb=%d variant=%d: %u B LDS does not fit\n", b, v, bytes);
+            continue;
+        }
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m->fn),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        launch_raw(m->fn, b, bytes, dA, dB, dW, dU, dI, T, nullptr);  // warm-up
+        std::vector<double> samples;
+        for (int s = 0; s < 5; ++s) {
+            (void)hipEventRecord(e0, nullptr);
+            launch_raw(m->fn, b, bytes, dA, dB, dW, dU, dI, T, nullptr);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            samples.push_back(T / (ms * 1e-3));
+        }
+        if (hipDeviceSynchronize() != hipSuccess) {
+            cleanup();
+            return fail(GENEIG_E_HIP, "autotune: kernel failed for bpb=%d", b);
+        }
+        (void)hipMemcpy(Wc.data(), dW, Wc.size() * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(Uc.data(), dU, Uc.size() * 16, hipMemcpyDeviceToHost);
+        const double mps = median(samples);
+        bool ok = true;
+        double eig = 0.0, phase = 0.0;
+        if (!have) {
+            Wref = Wc;
+            Uref = Uc;
+            have = true;
+        } else {
+            for (int k = 0; k < SPOT; ++k) {
+                const size_t p = (size_t)k * (T / SPOT);
+                for (int i = 0; i < n; ++i) {
+                    const double wr = Wref[p * n + i];
+                    eig = std::max(eig, std::fabs(Wc[p * n + i] - wr) / std::max(std::fabs(wr), 1.0));
+                    std::complex<double> inner = 0.0;
+                    for (int r = 0; r < n; ++r) {
+                        std::complex<double> bu = 0.0;
+                        for (int c = 0; c < n; ++c) bu += B[nn * p + r + (size_t)c * n] * Uref[nn * p + c + (size_t)i * n];
+                        inner += std::conj(Uc[nn * p + r + (size_t)i * n]) * bu;
+                    }
+                    phase = std::max(phase, 1.0 - std::abs(inner));
+                }
+            }
+            ok = eig < tol && phase < tol;
+        }
+        std::fprintf(stderr, "[autotune] n=%d bpb=%d variant=%d: median %.0f pairs/s eig=%.2e phase=%.2e %s\n",
+                     n, b, v, mps, eig, phase, ok ? "[OK]" : "[REJECT correctness]");
+        if (ok && mps > win.pairs_per_s) {
+            win.bpb = b;
+            win.variant = v;
+            win.pairs_per_s = mps;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cleanup();
+    if (win.bpb == 0) return fail(GENEIG_E_TUNE, "autotune: no correctness-validated candidate for n=%d", n);
+    sweep_ran = true;
+    return GENEIG_E_OK;
+}
+
+extern "C" {
+
+int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_request,
+                         int force_block_dim_x, int tuning_mode, int cache_mode) {
+    if (!out) return fail(GENEIG_E_ARG, "null output handle");
+    *out = nullptr;
+    if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
+    if (force_block_dim_x > 0 && force_block_dim_x != 64)
+        return fail(GENEIG_E_FIT, "block_dim_x=%d: only one wavefront (64 threads) per pair is built",
+                    force_block_dim_x);
+    if (bpb_request > 16) return fail(GENEIG_E_ARG, "pairs per workgroup %d > 16", bpb_request);
+    if (cache_mode < 0 || cache_mode > 2) return fail(GENEIG_E_ARG, "bad cache mode %d", cache_mode);
+    std::unique_ptr<geneig_solver> s(new geneig_solver);
+    s->n = n;
+    s->device = device_id;
+    HIP_TRY(hipSetDevice(device_id));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device_id));
+    s->arch = prop.gcnArchName;
+    int v = 0;
+    HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device_id));
+    s->lds_per_cu = (size_t)v;
+    HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id));
+    s->lds_per_block = std::max((size_t)v, (size_t)0);
+    if (s->lds_per_cu < s->lds_per_block) s->lds_per_cu = s->lds_per_block;
+
+    const auto t0 = std::chrono::steady_clock::now();
+    int bpb = bpb_request, variant = 0;
+    if (tuning_mode == GENEIG_TUNING_AUTOTUNE && force_block_dim_x <= 0) {
+        TuneWinner win;
+        bool hit = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find({n, s->arch});
+            if (it != g_tune.end()) { win = it->second; hit = true; }
+        }
+        if (!hit && cache_mode == GENEIG_CACHE_AUTO) {
+            std::ifstream f(tune_path_str(n, s->arch));
+            if (f >> win.bpb >> win.variant >> win.pairs_per_s && win.bpb >= 1 && win.bpb <= 16) hit = true;
+            else win = TuneWinner();
+        }
+        if (bpb_request > 0 && hit && win.bpb != bpb_request) hit = false;
+        if (!hit) {
+            // probe the per-pair LDS need with a bpb=1 module, then sweep
+            std::shared_ptr<Module> m1;
+            const char* lay = nullptr;
+            int rc = s->acquire(1, 0, cache_mode, m1, &lay);
+            if (rc) return rc;
+            std::vector<std::pair<int, int>> cands;
+            for (int b : {1, 2, 4, 8})
+                if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_per_pair)) cands.push_back({b, 0});
+            if (cands.empty()) return fail(GENEIG_E_FIT, "n=%d: no pairs-per-workgroup candidate fits", n);
+            rc = s->autotune(cache_mode, cands, win);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[{n, s->arch}] = win;
+            if (cache_mode == GENEIG_CACHE_AUTO) {
+                char line[128];
+                int len = std::snprintf(line, sizeof line, "%d %d %.1f\n", win.bpb, win.variant, win.pairs_per_s);
+                write_atomic(tune_path_str(n, s->arch), line, (size_t)len);
+            }
+        }
+        bpb = win.bpb;
+        variant = win.variant;
+        s->was_autotuned = true;
+        s->autotuned_bdx = 64;
+    }
+    if (bpb <= 0) {
+        std::shared_ptr<Module> m1;
+        const char* lay = nullptr;
+        int rc = s->acquire(1, 0, cache_mode, m1, &lay);
+        if (rc) return rc;
+        bpb = s->suggest_bpb(m1->lds_per_pair);
+        if (bpb == 0)
+            return fail(GENEIG_E_FIT, "n=%d needs %u B LDS per pair, device allows %zu per workgroup", n,
+                        m1->lds_per_pair, s->lds_per_block);
+    }
+    const char* lay = "compile";
+    int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay);
+    if (rc) return rc;
+    s->acquire_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->layer = lay;
+    s->bpb = bpb;
+    s->variant = variant;
+    s->shared_bytes = s->module->lds_per_pair * (unsigned)bpb;
+    if (!s->fits(bpb, s->module->lds_per_pair))
+        return fail(GENEIG_E_FIT, "bpb=%d needs %u B LDS > %zu per workgroup", bpb, s->shared_bytes,
+                    s->lds_per_block);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(s->module->fn),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->shared_bytes);
+    *out = s.release();
+    return GENEIG_E_OK;
+}
+
+void geneig_solver_destroy(geneig_solver_t s) { delete s; }
+
+int geneig_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W, void* d_U,
+                         int* d_info, int batch_size, hipStream_t stream) {
+    if (!s) return fail(GENEIG_E_ARG, "null solver");
+    return s->launch_raw(s->module->fn, s->bpb, s->shared_bytes, d_H, d_S, d_W, d_U, d_info, batch_size,
+                         stream);
+}
+
+int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W,
+                                 void* d_U, int* d_info, int total, int chunk, int nstreams,
+                                 const hipStream_t* streams) {
+    if (!s) return fail(GENEIG_E_ARG, "null solver");
+    if (total <= 0) return GENEIG_E_OK;
+    if (chunk <= 0) return fail(GENEIG_E_ARG, "launch_chunked: chunk_size must be > 0");
+    if (nstreams <= 0 || !streams) return fail(GENEIG_E_ARG, "launch_chunked: need num_streams > 0 streams");
+    const size_t mat = (size_t)s->n * s->n * 16;
+    int ci = 0;
+    for (int off = 0; off < total; off += chunk, ++ci) {
+        const int bs = std::min(chunk, total - off);
+        int rc = geneig_solver_launch(s, (const char*)d_H + mat * off, (const char*)d_S + mat * off,
+                                      d_W + (size_t)s->n * off, (char*)d_U + mat * off, d_info + off, bs,
+                                      streams[ci % nstreams]);
+        if (rc) return rc;
+    }
+    return GENEIG_E_OK;
+}
+
+int geneig_solver_matrix_size(geneig_solver_t s) { return s ? s->n : 0; }
+int geneig_solver_batches_per_block(geneig_solver_t s) { return s ? s->bpb : 0; }
+void geneig_solver_block_dim(geneig_solver_t s, unsigned* x, unsigned* y, unsigned* z) {
+    if (x) *x = s ? 64u : 0u;
+    if (y) *y = s ? (unsigned)s->bpb : 0u;
+    if (z) *z = s ? 1u : 0u;
+}
+unsigned geneig_solver_shared_mem_bytes(geneig_solver_t s) { return s ? s->shared_bytes : 0; }
+int geneig_solver_autotuned_block_dim_x(geneig_solver_t s) { return s ? s->autotuned_bdx : 0; }
+int geneig_solver_was_autotuned(geneig_solver_t s) { return s ? (int)s->was_autotuned : 0; }
+int geneig_solver_autotune_sweep_ran(geneig_solver_t s) { return s ? (int)s->sweep_ran : 0; }
+const char* geneig_solver_cache_layer_used(geneig_solver_t s) { return s ? s->layer : ""; }
+double geneig_solver_acquire_module_ms(geneig_solver_t s) { return s ? s->acquire_ms : 0.0; }
+int geneig_solver_variant(geneig_solver_t s) { return s ? s->variant : -1; }
+
+size_t geneig_cache_dir(char* buf, size_t len) {
+    const std::string d = cache_dir_str();
+    if (buf && len) std::snprintf(buf, len, "%s", d.c_str());
+    return d.size();
+}
+
+size_t geneig_cache_path(int n, int bpb, const char* arch, int block_dim_x, int variant, char* buf,
+                         size_t len) {
+    const std::string p = cache_path_str(n, bpb, arch ? arch : "", block_dim_x, variant);
+    if (buf && len) std::snprintf(buf, len, "%s", p.c_str());
+    return p.size();
+}
+
+int geneig_rtc_compile(int n, int bpb, int variant, const char* arch, void* buf, size_t cap, size_t* size) {
+    std::vector<char> code;
+    int rc = rtc_compile(n, bpb, variant, arch ? arch : "gfx950", code);
+    if (rc) return rc;
+    if (size) *size = code.size();
+    if (buf && cap >= code.size()) std::memcpy(buf, code.data(), code.size());
+    return GENEIG_E_OK;
+}
+
+uint64_t geneig_source_hash(void) { return source_hash(); }
+uint64_t geneig_fnv1a64(const void* data, size_t len, uint64_t seed) {
+    return fnv1a(data, len, seed ? seed : kFnvOffset);
+}
+
+int geneig_make_streams(int n, hipStream_t* out) {
+    for (int i = 0; i < n; ++i) HIP_TRY(hipStreamCreateWithFlags(&out[i], hipStreamNonBlocking));
+    return GENEIG_E_OK;
+}
+void geneig_destroy_streams(int n, hipStream_t* streams) {
+    for (int i = 0; i < n; ++i)
+        if (streams[i]) (void)hipStreamDestroy(streams[i]);
+}
+
+const char* geneig_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

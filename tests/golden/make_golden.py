@@ -1,0 +1,56 @@
+"""Writes the committed golden fixtures under tests/golden/ (GEIG v1 format).
+
+Run in the build container (needs /root/reference for the ref_* files and a
+LAPACK for the own_* files):  python tests/golden/make_golden.py
+
+* ref_n{N}_{mode}.bin -- the first pairs of the reference's own fixtures
+  tests/data/n{N}_b8_s42_{random,nearid}.bin (inputs from its mt19937_64
+  generators, outputs from LAPACKE_zhegvd; tests/lapack_reference.cpp:154-296),
+  kept as data so the parity tests can run where the reference is absent.
+* own_*.bin -- this project's synthetic families (the oracle's counter-based
+  generators) with eigenpairs from LAPACK zhegv (MKL or scipy's OpenBLAS).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+REF = "/root/reference/tests/data"
+# pairs kept per reference fixture (smaller n keep more)
+KEEP = {16: 2, 32: 2, 48: 1, 54: 2, 64: 1, 96: 1}
+
+
+def main():
+    if os.path.isdir(REF):
+        for n, keep in KEEP.items():
+            for mode in ("random", "nearid"):
+                src = os.path.join(REF, "n%d_b8_s42_%s.bin" % (n, mode))
+                fx = O.geig_read(src, batch=keep)
+                dst = os.path.join(HERE, "ref_n%d_%s.bin" % (n, mode))
+                O.geig_write(dst, fx["H"], fx["S"], fx["W"], fx["U"], seed=fx["seed"])
+                print("wrote", dst, keep)
+    else:
+        print("reference fixtures not present; ref_* files left as they are")
+    if O.lapack_load() is None:
+        raise SystemExit("no LAPACK available for the own_* fixtures")
+    own = [
+        ("own_spinwave_n54.bin", lambda: O.gen_spinwave(27, 4, seed=2024, first_q=0)),
+        ("own_generic_n32.bin", lambda: O.gen_generic(32, 4, seed=42)),
+        ("own_generic_n8.bin", lambda: O.gen_generic(8, 8, seed=7)),
+        ("own_generic_n96.bin", lambda: O.gen_generic(96, 1, seed=96)),
+    ]
+    for name, fn in own:
+        A, B = fn()
+        W, X, info = O.lapack_zhegv(A, B)
+        assert int(np.abs(info).max()) == 0
+        O.geig_write(os.path.join(HERE, name), A, B, W, X, seed=0)
+        print("wrote", name, A.shape)
+
+
+if __name__ == "__main__":
+    main()

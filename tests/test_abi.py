@@ -1,0 +1,112 @@
+"""CPU tests of the product library: it loads, exports every symbol the
+public headers declare, compiles the runtime-n kernel with hipRTC without a
+device, and its cache/host logic behaves as the reference's
+(src/nvrtc/nvrtc_solver.cpp:321-431).  No compute runs here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("geneig.h", "geneig_solver.h", "geneig_gen.h")]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", txt, flags=re.M):
+            if m.group(1) not in ("defined",):
+                names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_headers_declare_entry_points():
+    names = declared_functions()
+    for must in ("geneig_full_n54_batched_launch", "chol_n54_launch", "geneig_finish_n54_launch",
+                 "geneig_solver_create", "geneig_solver_launch_chunked", "geneig_cache_path"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(geneig):
+    L = geneig.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_headers_compile_as_c(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text("".join('#include "%s"\n' % h for h in HEADERS) + "int main(void){return 0;}\n")
+    subprocess.check_call(["gcc", "-std=c11", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                           "-c", str(src), "-o", str(tmp_path / "t.o")])
+
+
+def test_static_queries(geneig):
+    L = geneig.lib()
+    assert L.geneig_full_n54_batched_block_dim_x() == 64
+    lds = L.geneig_full_n54_batched_shared_mem_bytes()
+    assert 16 * 54 * 55 <= lds <= 160 * 1024
+    assert L.geneig_reduce_eig_n54_chol_smem_bytes() + L.geneig_reduce_eig_n54_heev_smem_bytes() == \
+        L.geneig_reduce_eig_n54_shared_mem_bytes()
+
+
+@pytest.mark.parametrize("n", [1, 2, 8, 16, 32, 48, 54, 64, 96])
+def test_rtc_compiles_without_device(geneig, n):
+    code = geneig.rtc_compile(n, 1)
+    assert code[:4] == b"\x7fELF" and len(code) > 1024
+
+
+def test_rtc_rejects_bad_arguments(geneig):
+    with pytest.raises(geneig.GeneigError):
+        geneig.rtc_compile(0)
+    with pytest.raises(geneig.GeneigError):
+        geneig.rtc_compile(200)
+    with pytest.raises(geneig.GeneigError):
+        geneig.rtc_compile(32, 1, variant=7)
+
+
+def test_cache_dir_resolution(geneig, monkeypatch):
+    monkeypatch.setenv("SUNNY_GENEIG_CACHE_DIR", "/x/y")
+    assert geneig.GeneigSolver.cache_dir() == "/x/y"
+    monkeypatch.delenv("SUNNY_GENEIG_CACHE_DIR")
+    monkeypatch.setenv("XDG_CACHE_HOME", "/xdg")
+    assert geneig.GeneigSolver.cache_dir() == "/xdg/sunny_geneig/hsaco"
+    monkeypatch.delenv("XDG_CACHE_HOME")
+    monkeypatch.setenv("HOME", "/home/u")
+    assert geneig.GeneigSolver.cache_dir() == "/home/u/.cache/sunny_geneig/hsaco"
+
+
+def test_cache_path_key(geneig, monkeypatch):
+    monkeypatch.setenv("SUNNY_GENEIG_CACHE_DIR", "/c")
+    p = geneig.GeneigSolver.cache_path(54, 2, "gfx950:sramecc+:xnack-", 64, 0)
+    src = geneig.lib().geneig_source_hash()
+    assert re.fullmatch(r"/c/54_2_gfx950_sramecc__xnack-_64_v0_lib[0-9a-f]{16}_src%016x\.hsaco" % src, p), p
+    assert p != geneig.GeneigSolver.cache_path(54, 1, "gfx950:sramecc+:xnack-", 64, 0)
+
+
+def test_source_hash_is_fnv1a_of_device_source(geneig, oracle):
+    src = open(os.path.join(ROOT, "sunny-batched-geneig_amd", "csrc", "geneig_device.inc"), "rb").read()
+    assert geneig.lib().geneig_source_hash() == oracle.fnv1a64(src)
+    buf = C.create_string_buffer(b"foobar", 6)
+    assert geneig.lib().geneig_fnv1a64(C.cast(buf, C.c_void_p), 6, 0) == 0x85944171f73967e8
+
+
+def test_solver_create_reports_errors(geneig):
+    with pytest.raises(geneig.GeneigError, match="outside the supported range"):
+        geneig.GeneigSolver(0)
+    with pytest.raises(geneig.GeneigError, match="64 threads"):
+        geneig.GeneigSolver(54, force_block_dim_x=128)
+
+
+def test_product_does_not_touch_oracle():
+    """The product sources never reference the oracle (it is the checker only)."""
+    pkg = os.path.join(ROOT, "sunny-batched-geneig_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".inc", ".h", "Makefile")) and "_build" not in dirpath:
+                txt = open(os.path.join(dirpath, f), errors="replace").read()
+                assert "oracle" not in txt.lower(), f
