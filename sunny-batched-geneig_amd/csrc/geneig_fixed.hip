@@ -7,79 +7,101 @@
 // reference returned void; callers that ignore the result keep working.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
+#include <set>
 
 #include "geneig_device.inc"
+#include "geneig_ws.hpp"
 #include "../../include/geneig.h"
 
 namespace {
 
 constexpr int kN = 54;
 using C54 = geneig::Cfg<kN>;
-// The batched launcher packs this many independent pairs (one wave each)
-// into a workgroup; 1 keeps the LDS allocation per workgroup at one pair so
-// the hardware can place three workgroups on a CU.
-constexpr int kBatchedBPB = 1;
+using W54 = geneig::Ws<kN>;
+// K1/K3 pack this many independent pairs (one wave each) into a workgroup; 1
+// keeps the LDS allocation per workgroup at one pair so three workgroups fit
+// a CU.
+constexpr int kBPB = 1;
 
 template <int MODE>
-__global__ void __launch_bounds__(64) geneig_single_kernel(const geneig::cplx* __restrict__ A,
-                                                           const geneig::cplx* __restrict__ B,
-                                                           double* __restrict__ W,
-                                                           geneig::cplx* __restrict__ U,
-                                                           int* __restrict__ info) {
+__global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __restrict__ A,
+                                                        const geneig::cplx* __restrict__ B,
+                                                        double* __restrict__ W, geneig::cplx* __restrict__ U,
+                                                        int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::solve_pair<kN, MODE>(A, B, W, U, info, smem, threadIdx.x);
+    geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, smem);
 }
 
-template <int MODE, int BPB>
-__global__ void __launch_bounds__(64 * BPB) geneig_batched_kernel(const geneig::cplx* __restrict__ A,
-                                                                  const geneig::cplx* __restrict__ B,
-                                                                  double* __restrict__ W,
-                                                                  geneig::cplx* __restrict__ U,
-                                                                  int* __restrict__ info, int batch) {
+__global__ void __launch_bounds__(64 * C54::R) k2_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::batched<kN, BPB, MODE>(A, B, W, U, info, batch, smem);
+    geneig::k2_batched<kN>(info, ws, batch, smem);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64 * kBPB) k3_kernel(const geneig::cplx* __restrict__ B,
+                                                        double* __restrict__ W, geneig::cplx* __restrict__ U,
+                                                        const int* __restrict__ info, unsigned char* ws,
+                                                        int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    geneig::k3_batched<kN, kBPB, MODE>(B, W, U, info, ws, batch, smem);
 }
 
 // hipFuncSetAttribute once per kernel per process, thread-safe (the reference
-// used an unguarded static bool, src/geneig_fixed.cu:138-144).
-template <typename K>
-hipError_t set_lds_once(std::once_flag& flag, hipError_t& status, K kernel, unsigned bytes) {
-    std::call_once(flag, [&] {
-        status = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    });
-    return status;
+// used an unguarded static bool per launcher, src/geneig_fixed.cu:138-144).
+std::mutex g_attr_mu;
+std::set<const void*> g_attr_done;
+
+hipError_t set_lds_once(const void* kernel, unsigned bytes) {
+    std::lock_guard<std::mutex> lk(g_attr_mu);
+    if (g_attr_done.count(kernel)) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) g_attr_done.insert(kernel);
+    return e;
 }
 
-template <int MODE>
-hipError_t launch_single(const void* A, const void* B, double* W, void* U, int* info,
-                         hipStream_t stream) {
-    static std::once_flag flag;
-    static hipError_t st = hipSuccess;
-    auto k = geneig_single_kernel<MODE>;
-    hipError_t e = set_lds_once(flag, st, k, C54::BYTES);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(1), dim3(64), C54::BYTES, stream,
-                       static_cast<const geneig::cplx*>(A), static_cast<const geneig::cplx*>(B), W,
-                       static_cast<geneig::cplx*>(U), info);
-    return hipGetLastError();
-}
+#define RETURN_IF(expr)                      \
+    do {                                     \
+        hipError_t _e = (expr);              \
+        if (_e != hipSuccess) return _e;     \
+    } while (0)
 
+// The solve pipeline over a batch, chunked to bound the workspace.
+// MODE_FINISH takes M in A and L in B.
 template <int MODE>
-hipError_t launch_batched(const void* A, const void* B, double* W, void* U, int* info, int batch,
-                          hipStream_t stream) {
-    static std::once_flag flag;
-    static hipError_t st = hipSuccess;
-    auto k = geneig_batched_kernel<MODE, kBatchedBPB>;
-    hipError_t e = set_lds_once(flag, st, k, C54::BYTES * kBatchedBPB);
-    if (e != hipSuccess) return e;
+hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int batch, hipStream_t stream) {
+    auto k1 = k1_kernel<MODE>;
+    auto k3 = k3_kernel<MODE>;
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), C54::BYTES * kBPB));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2_kernel), C54::K2_BYTES * C54::R));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), C54::BYTES * kBPB));
     if (batch <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)((batch + kBatchedBPB - 1) / kBatchedBPB);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64, kBatchedBPB), C54::BYTES * kBatchedBPB, stream,
-                       static_cast<const geneig::cplx*>(A), static_cast<const geneig::cplx*>(B), W,
-                       static_cast<geneig::cplx*>(U), info, batch);
-    return hipGetLastError();
+    constexpr bool eig = (MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG ||
+                          MODE == geneig::MODE_FINISH);
+    const int chunk = geneig_host::chunk_pairs(batch, W54::PAIR_BYTES);
+    void* ws = nullptr;
+    if (eig) RETURN_IF(geneig_host::workspace(stream, (size_t)chunk * W54::PAIR_BYTES, &ws));
+    const size_t mat = (size_t)kN * kN;
+    const auto* a = static_cast<const geneig::cplx*>(A);
+    const auto* b = static_cast<const geneig::cplx*>(B);
+    auto* u = static_cast<geneig::cplx*>(U);
+    for (int off = 0; off < batch; off += chunk) {
+        const int P = std::min(chunk, batch - off);
+        const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
+        unsigned char* w = static_cast<unsigned char*>(ws);
+        hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), C54::BYTES * kBPB, stream, a + mat * off,
+                           b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
+        RETURN_IF(hipGetLastError());
+        if (!eig) continue;
+        hipLaunchKernelGGL(k2_kernel, dim3((unsigned)P), dim3(64 * C54::R), C54::K2_BYTES * C54::R, stream,
+                           info + off, w, P);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), C54::BYTES * kBPB, stream, b + mat * off,
+                           W + (size_t)kN * off, u + mat * off, info + off, w, P);
+        RETURN_IF(hipGetLastError());
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -88,14 +110,14 @@ extern "C" {
 
 int chol_n54_launch(const hipDoubleComplex* d_S, hipDoubleComplex* d_L, int* d_info,
                     hipStream_t stream) {
-    return launch_single<geneig::MODE_CHOL>(nullptr, d_S, nullptr, d_L, d_info, stream);
+    return run<geneig::MODE_CHOL>(nullptr, d_S, nullptr, d_L, d_info, 1, stream);
 }
 unsigned int chol_n54_block_dim_x(void) { return 64; }
 unsigned int chol_n54_shared_mem_bytes(void) { return C54::BYTES; }
 
 int geneig_reduce_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                              hipDoubleComplex* d_M, int* d_info, hipStream_t stream) {
-    return launch_single<geneig::MODE_REDUCE>(d_H, d_S, nullptr, d_M, d_info, stream);
+    return run<geneig::MODE_REDUCE>(d_H, d_S, nullptr, d_M, d_info, 1, stream);
 }
 unsigned int geneig_reduce_n54_block_dim_x(void) { return 64; }
 unsigned int geneig_reduce_n54_shared_mem_bytes(void) { return C54::BYTES; }
@@ -103,7 +125,7 @@ unsigned int geneig_reduce_n54_shared_mem_bytes(void) { return C54::BYTES; }
 int geneig_reduce_eig_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                                  double* d_W, hipDoubleComplex* d_V, int* d_info,
                                  hipStream_t stream) {
-    return launch_single<geneig::MODE_REDUCE_EIG>(d_H, d_S, d_W, d_V, d_info, stream);
+    return run<geneig::MODE_REDUCE_EIG>(d_H, d_S, d_W, d_V, d_info, 1, stream);
 }
 unsigned int geneig_reduce_eig_n54_block_dim_x(void) { return 64; }
 unsigned int geneig_reduce_eig_n54_shared_mem_bytes(void) { return C54::BYTES; }
@@ -115,7 +137,7 @@ unsigned int geneig_reduce_eig_n54_heev_workspace(void) { return 2 * C54::NP; }
 
 int geneig_full_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, double* d_W,
                            hipDoubleComplex* d_U, int* d_info, hipStream_t stream) {
-    return launch_single<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, stream);
+    return run<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, 1, stream);
 }
 unsigned int geneig_full_n54_block_dim_x(void) { return 64; }
 unsigned int geneig_full_n54_shared_mem_bytes(void) { return C54::BYTES; }
@@ -123,17 +145,17 @@ unsigned int geneig_full_n54_shared_mem_bytes(void) { return C54::BYTES; }
 int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                                    double* d_W, hipDoubleComplex* d_U, int* d_info,
                                    int batch_size, hipStream_t stream) {
-    return launch_batched<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, batch_size, stream);
+    return run<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, batch_size, stream);
 }
 unsigned int geneig_full_n54_batched_block_dim_x(void) { return 64; }
-unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return C54::BYTES * kBatchedBPB; }
+unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return C54::BYTES * kBPB; }
 
 int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex* d_M, double* d_W,
                              hipDoubleComplex* d_U, int* d_info, int batch_size,
                              hipStream_t stream) {
-    return launch_batched<geneig::MODE_FINISH>(d_M, d_L, d_W, d_U, d_info, batch_size, stream);
+    return run<geneig::MODE_FINISH>(d_M, d_L, d_W, d_U, d_info, batch_size, stream);
 }
 unsigned int geneig_finish_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_finish_n54_shared_mem_bytes(void) { return C54::BYTES * kBatchedBPB; }
+unsigned int geneig_finish_n54_shared_mem_bytes(void) { return C54::BYTES * kBPB; }
 
 }  // extern "C"

@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../../include/geneig_solver.h"
+#include "geneig_ws.hpp"
 #include "geneig_device_src.inc"  // kGeneigDeviceSource: geneig_device.inc as a string
 
 namespace {
@@ -189,8 +190,12 @@ int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vecto
 // ---- loaded module -----------------------------------------------------------
 struct Module {
     hipModule_t mod = nullptr;
-    hipFunction_t fn = nullptr;
-    unsigned lds_per_pair = 0;
+    hipFunction_t fn = nullptr;    // K1: factor, reduce, tridiagonalize, form Q
+    hipFunction_t fn2 = nullptr;   // K2: QL with the rotation product in registers
+    hipFunction_t fn3 = nullptr;   // K3: sort, Y = Q Z, back-transform
+    unsigned lds_per_pair = 0;     // K1 / K3 LDS bytes per pair
+    unsigned k2_lds = 0, k2_threads = 0;
+    unsigned long long ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
     Module() = default;
     Module(const Module&) = delete;
@@ -207,12 +212,22 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     m->bpb = bpb;
     m->variant = variant;
     HIP_TRY(hipModuleLoadData(&m->mod, code.data()));
-    HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_full_kernel"));
-    hipDeviceptr_t ptr = nullptr;
-    size_t size = 0;
-    HIP_TRY(hipModuleGetGlobal(&ptr, &size, m->mod, "geneig_lds_bytes_per_pair"));
-    if (size != sizeof(unsigned)) return fail(GENEIG_E_HIP, "geneig_lds_bytes_per_pair has size %zu", size);
-    HIP_TRY(hipMemcpyDtoH(&m->lds_per_pair, ptr, size));
+    HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_k1"));
+    HIP_TRY(hipModuleGetFunction(&m->fn2, m->mod, "geneig_k2"));
+    HIP_TRY(hipModuleGetFunction(&m->fn3, m->mod, "geneig_k3"));
+    auto read_global = [&](const char* name, void* dst, size_t want) -> int {
+        hipDeviceptr_t ptr = nullptr;
+        size_t size = 0;
+        HIP_TRY(hipModuleGetGlobal(&ptr, &size, m->mod, name));
+        if (size != want) return fail(GENEIG_E_HIP, "%s has size %zu, expected %zu", name, size, want);
+        HIP_TRY(hipMemcpyDtoH(dst, ptr, size));
+        return GENEIG_E_OK;
+    };
+    int rc = read_global("geneig_lds_bytes_per_pair", &m->lds_per_pair, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_k2_lds_bytes", &m->k2_lds, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_k2_threads", &m->k2_threads, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
+    if (rc) return rc;
     out = m;
     return GENEIG_E_OK;
 }
@@ -367,15 +382,38 @@ struct geneig_solver {
         return best;
     }
 
-    int launch_raw(hipFunction_t fn, int b, unsigned bytes, const void* H, const void* S, double* W,
-                   void* U, int* info, int batch, hipStream_t stream) {
+    // K1 -> K2 -> K3 over the batch, chunked to bound the per-stream workspace.
+    int launch_raw(const Module& m, int b, const void* H, const void* S, double* W, void* U, int* info,
+                   int batch, hipStream_t stream) {
         if (batch <= 0) return GENEIG_E_OK;
-        const void* h = H;
-        const void* s = S;
-        void* args[] = {&h, &s, &W, &U, &info, &batch};
-        const unsigned grid = (unsigned)((batch + b - 1) / b);
-        HIP_TRY(hipModuleLaunchKernel(fn, grid, 1, 1, 64, (unsigned)b, 1, bytes, stream, args, nullptr));
+        const int chunk = geneig_host::chunk_pairs(batch, m.ws_pair_bytes);
+        void* ws = nullptr;
+        HIP_TRY(geneig_host::workspace(stream, (size_t)chunk * m.ws_pair_bytes, &ws));
+        const size_t mat = (size_t)n * n * 16;
+        const unsigned bytes = m.lds_per_pair * (unsigned)b;
+        for (int off = 0; off < batch; off += chunk) {
+            int P = std::min(chunk, batch - off);
+            const void* h = static_cast<const char*>(H) + mat * off;
+            const void* sp = static_cast<const char*>(S) + mat * off;
+            double* w = W + (size_t)n * off;
+            void* u = static_cast<char*>(U) + mat * off;
+            int* inf = info + off;
+            void* a1[] = {&h, &sp, &w, &u, &inf, &ws, &P};
+            const unsigned g13 = (unsigned)((P + b - 1) / b);
+            HIP_TRY(hipModuleLaunchKernel(m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1, nullptr));
+            void* a2[] = {&inf, &ws, &P};
+            HIP_TRY(hipModuleLaunchKernel(m.fn2, (unsigned)P, 1, 1, m.k2_threads, 1, 1, m.k2_lds * (m.k2_threads / 64),
+                                          stream, a2, nullptr));
+            void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
+            HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a3, nullptr));
+        }
         return GENEIG_E_OK;
+    }
+
+    void set_attrs(const Module& m, int b) {
+        const int bytes = (int)(m.lds_per_pair * (unsigned)b);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn3), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     }
 
     int autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands, TuneWinner& win);
@@ -424,13 +462,15 @@ int geneig_solver::autotune(int cache_mode, const std::vector<std::pair<int, int
             std::fprintf(stderr, "[autotune] bpb=%d variant=%d: %u B LDS does not fit\n", b, v, bytes);
             continue;
         }
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m->fn),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        launch_raw(m->fn, b, bytes, dA, dB, dW, dU, dI, T, nullptr);  // warm-up
+        set_attrs(*m, b);
+        if (launch_raw(*m, b, dA, dB, dW, dU, dI, T, nullptr) != GENEIG_E_OK) {  // warm-up
+            std::fprintf(stderr, "[autotune] bpb=%d variant=%d: launch failed\n", b, v);
+            continue;
+        }
         std::vector<double> samples;
         for (int s = 0; s < 5; ++s) {
             (void)hipEventRecord(e0, nullptr);
-            launch_raw(m->fn, b, bytes, dA, dB, dW, dU, dI, T, nullptr);
+            (void)launch_raw(*m, b, dA, dB, dW, dU, dI, T, nullptr);
             (void)hipEventRecord(e1, nullptr);
             (void)hipEventSynchronize(e1);
             float ms = 0.f;
@@ -571,8 +611,7 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
     if (!s->fits(bpb, s->module->lds_per_pair))
         return fail(GENEIG_E_FIT, "bpb=%d needs %u B LDS > %zu per workgroup", bpb, s->shared_bytes,
                     s->lds_per_block);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(s->module->fn),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->shared_bytes);
+    s->set_attrs(*s->module, bpb);
     *out = s.release();
     return GENEIG_E_OK;
 }
@@ -582,8 +621,7 @@ void geneig_solver_destroy(geneig_solver_t s) { delete s; }
 int geneig_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W, void* d_U,
                          int* d_info, int batch_size, hipStream_t stream) {
     if (!s) return fail(GENEIG_E_ARG, "null solver");
-    return s->launch_raw(s->module->fn, s->bpb, s->shared_bytes, d_H, d_S, d_W, d_U, d_info, batch_size,
-                         stream);
+    return s->launch_raw(*s->module, s->bpb, d_H, d_S, d_W, d_U, d_info, batch_size, stream);
 }
 
 int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W,
