@@ -31,12 +31,13 @@ __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __res
                                                         double* __restrict__ W, geneig::cplx* __restrict__ U,
                                                         int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, smem);
+    __shared__ geneig::cplx vecs[kBPB * C54::VEC];
+    geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, smem, vecs);
 }
 
-__global__ void __launch_bounds__(64 * C54::R) k2_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2_batched<kN>(info, ws, batch, smem);
+__global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+    __shared__ geneig::cplx rotlog[C54::R * C54::K2_VEC];
+    geneig::k2_batched<kN>(info, ws, batch, rotlog);
 }
 
 template <int MODE>
@@ -45,7 +46,8 @@ __global__ void __launch_bounds__(64 * kBPB) k3_kernel(const geneig::cplx* __res
                                                         const int* __restrict__ info, unsigned char* ws,
                                                         int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k3_batched<kN, kBPB, MODE>(B, W, U, info, ws, batch, smem);
+    __shared__ geneig::cplx vecs[kBPB * C54::VEC];
+    geneig::k3_batched<kN, kBPB, MODE>(B, W, U, info, ws, batch, smem, vecs);
 }
 
 // hipFuncSetAttribute once per kernel per process, thread-safe (the reference
@@ -74,7 +76,7 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
     auto k1 = k1_kernel<MODE>;
     auto k3 = k3_kernel<MODE>;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), C54::BYTES * kBPB));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2_kernel), C54::K2_BYTES * C54::R));
+
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), C54::BYTES * kBPB));
     if (batch <= 0) return hipSuccess;
     constexpr bool eig = (MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG ||
@@ -94,7 +96,7 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
                            b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
         if (!eig) continue;
-        hipLaunchKernelGGL(k2_kernel, dim3((unsigned)P), dim3(64 * C54::R), C54::K2_BYTES * C54::R, stream,
+        hipLaunchKernelGGL(k2_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream,
                            info + off, w, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), C54::BYTES * kBPB, stream, b + mat * off,
@@ -113,14 +115,14 @@ int chol_n54_launch(const hipDoubleComplex* d_S, hipDoubleComplex* d_L, int* d_i
     return run<geneig::MODE_CHOL>(nullptr, d_S, nullptr, d_L, d_info, 1, stream);
 }
 unsigned int chol_n54_block_dim_x(void) { return 64; }
-unsigned int chol_n54_shared_mem_bytes(void) { return C54::BYTES; }
+unsigned int chol_n54_shared_mem_bytes(void) { return C54::TOTAL; }
 
 int geneig_reduce_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                              hipDoubleComplex* d_M, int* d_info, hipStream_t stream) {
     return run<geneig::MODE_REDUCE>(d_H, d_S, nullptr, d_M, d_info, 1, stream);
 }
 unsigned int geneig_reduce_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_reduce_n54_shared_mem_bytes(void) { return C54::BYTES; }
+unsigned int geneig_reduce_n54_shared_mem_bytes(void) { return C54::TOTAL; }
 
 int geneig_reduce_eig_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                                  double* d_W, hipDoubleComplex* d_V, int* d_info,
@@ -128,19 +130,19 @@ int geneig_reduce_eig_n54_launch(const hipDoubleComplex* d_H, const hipDoubleCom
     return run<geneig::MODE_REDUCE_EIG>(d_H, d_S, d_W, d_V, d_info, 1, stream);
 }
 unsigned int geneig_reduce_eig_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_reduce_eig_n54_shared_mem_bytes(void) { return C54::BYTES; }
+unsigned int geneig_reduce_eig_n54_shared_mem_bytes(void) { return C54::TOTAL; }
 // The factor lives in the U/V buffer, so the "Cholesky region" of LDS is the
-// working matrix itself; the eigen-stage region adds the two broadcast vectors.
-unsigned int geneig_reduce_eig_n54_chol_smem_bytes(void) { return C54::OFF_COL; }
-unsigned int geneig_reduce_eig_n54_heev_smem_bytes(void) { return C54::BYTES - C54::OFF_COL; }
-unsigned int geneig_reduce_eig_n54_heev_workspace(void) { return 2 * C54::NP; }
+// working matrix itself; the eigen-stage region is the two broadcast vectors.
+unsigned int geneig_reduce_eig_n54_chol_smem_bytes(void) { return C54::BYTES; }
+unsigned int geneig_reduce_eig_n54_heev_smem_bytes(void) { return C54::TOTAL - C54::BYTES; }
+unsigned int geneig_reduce_eig_n54_heev_workspace(void) { return C54::VEC; }
 
 int geneig_full_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, double* d_W,
                            hipDoubleComplex* d_U, int* d_info, hipStream_t stream) {
     return run<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, 1, stream);
 }
 unsigned int geneig_full_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_full_n54_shared_mem_bytes(void) { return C54::BYTES; }
+unsigned int geneig_full_n54_shared_mem_bytes(void) { return C54::TOTAL; }
 
 int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                                    double* d_W, hipDoubleComplex* d_U, int* d_info,
@@ -148,7 +150,7 @@ int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleC
     return run<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, batch_size, stream);
 }
 unsigned int geneig_full_n54_batched_block_dim_x(void) { return 64; }
-unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return C54::BYTES * kBPB; }
+unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return C54::TOTAL * kBPB; }
 
 int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex* d_M, double* d_W,
                              hipDoubleComplex* d_U, int* d_info, int batch_size,
@@ -156,6 +158,6 @@ int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex
     return run<geneig::MODE_FINISH>(d_M, d_L, d_W, d_U, d_info, batch_size, stream);
 }
 unsigned int geneig_finish_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_finish_n54_shared_mem_bytes(void) { return C54::BYTES * kBPB; }
+unsigned int geneig_finish_n54_shared_mem_bytes(void) { return C54::TOTAL * kBPB; }
 
 }  // extern "C"

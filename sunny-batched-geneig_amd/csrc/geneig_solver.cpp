@@ -193,8 +193,9 @@ struct Module {
     hipFunction_t fn = nullptr;    // K1: factor, reduce, tridiagonalize, form Q
     hipFunction_t fn2 = nullptr;   // K2: QL with the rotation product in registers
     hipFunction_t fn3 = nullptr;   // K3: sort, Y = Q Z, back-transform
-    unsigned lds_per_pair = 0;     // K1 / K3 LDS bytes per pair
-    unsigned k2_lds = 0, k2_threads = 0;
+    unsigned lds_per_pair = 0;     // K1 / K3 dynamic LDS bytes per pair
+    unsigned lds_total = 0;        // K1 / K3 dynamic + static LDS bytes per pair
+    unsigned k2_threads = 0;
     unsigned long long ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
     Module() = default;
@@ -224,7 +225,7 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
         return GENEIG_E_OK;
     };
     int rc = read_global("geneig_lds_bytes_per_pair", &m->lds_per_pair, sizeof(unsigned));
-    if (!rc) rc = read_global("geneig_k2_lds_bytes", &m->k2_lds, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_lds_total_per_pair", &m->lds_total, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_k2_threads", &m->k2_threads, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
     if (rc) return rc;
@@ -402,8 +403,7 @@ struct geneig_solver {
             const unsigned g13 = (unsigned)((P + b - 1) / b);
             HIP_TRY(hipModuleLaunchKernel(m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1, nullptr));
             void* a2[] = {&inf, &ws, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.fn2, (unsigned)P, 1, 1, m.k2_threads, 1, 1, m.k2_lds * (m.k2_threads / 64),
-                                          stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.fn2, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
             HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a3, nullptr));
         }
@@ -457,8 +457,8 @@ int geneig_solver::autotune(int cache_mode, const std::vector<std::pair<int, int
             std::fprintf(stderr, "[autotune] bpb=%d variant=%d: rejected at compile/load\n", b, v);
             continue;
         }
-        const unsigned bytes = m->lds_per_pair * (unsigned)b;
-        if (!fits(b, m->lds_per_pair)) {
+        const unsigned bytes = m->lds_total * (unsigned)b;
+        if (!fits(b, m->lds_total)) {
             std::fprintf(stderr, "[autotune] bpb=%d variant=%d: %u B LDS does not fit\n", b, v, bytes);
             continue;
         }
@@ -573,7 +573,7 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
             if (rc) return rc;
             std::vector<std::pair<int, int>> cands;
             for (int b : {1, 2, 4, 8})
-                if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_per_pair)) cands.push_back({b, 0});
+                if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_total)) cands.push_back({b, 0});
             if (cands.empty()) return fail(GENEIG_E_FIT, "n=%d: no pairs-per-workgroup candidate fits", n);
             rc = s->autotune(cache_mode, cands, win);
             if (rc) return rc;
@@ -595,10 +595,10 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
         const char* lay = nullptr;
         int rc = s->acquire(1, 0, cache_mode, m1, &lay);
         if (rc) return rc;
-        bpb = s->suggest_bpb(m1->lds_per_pair);
+        bpb = s->suggest_bpb(m1->lds_total);
         if (bpb == 0)
             return fail(GENEIG_E_FIT, "n=%d needs %u B LDS per pair, device allows %zu per workgroup", n,
-                        m1->lds_per_pair, s->lds_per_block);
+                        m1->lds_total, s->lds_per_block);
     }
     const char* lay = "compile";
     int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay);
@@ -607,8 +607,8 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
     s->layer = lay;
     s->bpb = bpb;
     s->variant = variant;
-    s->shared_bytes = s->module->lds_per_pair * (unsigned)bpb;
-    if (!s->fits(bpb, s->module->lds_per_pair))
+    s->shared_bytes = s->module->lds_total * (unsigned)bpb;
+    if (!s->fits(bpb, s->module->lds_total))
         return fail(GENEIG_E_FIT, "bpb=%d needs %u B LDS > %zu per workgroup", bpb, s->shared_bytes,
                     s->lds_per_block);
     s->set_attrs(*s->module, bpb);
