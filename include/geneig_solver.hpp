@@ -1,0 +1,111 @@
+// geneig_solver.hpp -- C++ interface of the runtime-n solver, header-only over
+// the C ABI of geneig_solver.h.  Same constructor arguments, launch methods,
+// introspection and statics as the reference's NvrtcGeneigSolver
+// (src/nvrtc/nvrtc_solver.hpp:81-217), with HIP types: device pointers are
+// void* (hipDeviceptr_t) instead of CUdeviceptr and streams are hipStream_t.
+// Errors throw std::runtime_error carrying the library's message, as the
+// reference does (src/nvrtc/nvrtc_solver.cpp:49-91).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "geneig_solver.h"
+
+namespace geneig {
+
+enum class TuningMode { Suggested = GENEIG_TUNING_SUGGESTED, Autotune = GENEIG_TUNING_AUTOTUNE };
+enum class CacheMode { Auto = GENEIG_CACHE_AUTO, NoDisk = GENEIG_CACHE_NODISK, NoCache = GENEIG_CACHE_NOCACHE };
+
+struct Dim3 {
+    unsigned x = 0, y = 0, z = 0;
+};
+
+// Cache key of one compiled module (nvrtc_solver.hpp:67-79): two solvers with
+// equal keys share one loaded module.
+struct ModuleCacheKey {
+    int n = 0;
+    int bpb = 0;
+    std::string arch;
+    int block_dim_x = 64;
+    int variant = 0;
+};
+
+inline void check(int rc, const char* what) {
+    if (rc != GENEIG_E_OK) throw std::runtime_error(std::string(what) + ": " + geneig_last_error());
+}
+
+class GeneigSolver {
+public:
+    explicit GeneigSolver(int n, int device_id, int batches_per_block_request = 0, int force_block_dim_x = 0,
+                          TuningMode tuning_mode = TuningMode::Suggested, CacheMode cache_mode = CacheMode::Auto) {
+        check(geneig_solver_create(&h_, n, device_id, batches_per_block_request, force_block_dim_x,
+                                   static_cast<int>(tuning_mode), static_cast<int>(cache_mode)),
+              "GeneigSolver");
+    }
+    ~GeneigSolver() { geneig_solver_destroy(h_); }
+    GeneigSolver(const GeneigSolver&) = delete;
+    GeneigSolver& operator=(const GeneigSolver&) = delete;
+    GeneigSolver(GeneigSolver&&) = delete;
+    GeneigSolver& operator=(GeneigSolver&&) = delete;
+
+    void launch(const void* d_H, const void* d_S, double* d_W, void* d_U, int* d_info, int batch_size,
+                hipStream_t stream) {
+        check(geneig_solver_launch(h_, d_H, d_S, d_W, d_U, d_info, batch_size, stream), "GeneigSolver::launch");
+    }
+    void launch_chunked(const void* d_H, const void* d_S, double* d_W, void* d_U, int* d_info,
+                        int total_matrices, int chunk_size, int num_streams,
+                        const std::vector<hipStream_t>& streams) {
+        if ((int)streams.size() != num_streams)
+            throw std::runtime_error("launch_chunked: streams.size() must equal num_streams");
+        check(geneig_solver_launch_chunked(h_, d_H, d_S, d_W, d_U, d_info, total_matrices, chunk_size,
+                                           num_streams, streams.data()),
+              "GeneigSolver::launch_chunked");
+    }
+
+    int matrix_size() const { return geneig_solver_matrix_size(h_); }
+    int batches_per_block() const { return geneig_solver_batches_per_block(h_); }
+    Dim3 block_dim() const {
+        Dim3 d;
+        geneig_solver_block_dim(h_, &d.x, &d.y, &d.z);
+        return d;
+    }
+    unsigned int shared_mem_bytes() const { return geneig_solver_shared_mem_bytes(h_); }
+    int autotuned_block_dim_x() const { return geneig_solver_autotuned_block_dim_x(h_); }
+    bool was_autotuned() const { return geneig_solver_was_autotuned(h_) != 0; }
+    bool autotune_sweep_ran() const { return geneig_solver_autotune_sweep_ran(h_) != 0; }
+    const char* cache_layer_used() const { return geneig_solver_cache_layer_used(h_); }
+    double acquire_module_ms() const { return geneig_solver_acquire_module_ms(h_); }
+
+    static std::string cache_dir() {
+        std::string s(geneig_cache_dir(nullptr, 0) + 1, '\0');
+        geneig_cache_dir(&s[0], s.size());
+        s.resize(s.size() - 1);
+        return s;
+    }
+    static std::string cubin_path(const ModuleCacheKey& key) {
+        const size_t len = geneig_cache_path(key.n, key.bpb, key.arch.c_str(), key.block_dim_x, key.variant,
+                                             nullptr, 0);
+        std::string s(len + 1, '\0');
+        geneig_cache_path(key.n, key.bpb, key.arch.c_str(), key.block_dim_x, key.variant, &s[0], s.size());
+        s.resize(len);
+        return s;
+    }
+
+private:
+    geneig_solver_t h_ = nullptr;
+};
+
+// make_streams / destroy_streams (nvrtc_solver.cpp:1109-1125).
+inline std::vector<hipStream_t> make_streams(int n) {
+    std::vector<hipStream_t> v(n, nullptr);
+    check(geneig_make_streams(n, v.data()), "make_streams");
+    return v;
+}
+inline void destroy_streams(std::vector<hipStream_t>& streams) {
+    geneig_destroy_streams((int)streams.size(), streams.data());
+    streams.clear();
+}
+
+}  // namespace geneig

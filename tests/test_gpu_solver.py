@@ -1,0 +1,187 @@
+"""GPU tests of the runtime-n solver contract (per-n hipRTC specialization,
+module cache, autotuner, pairs-per-workgroup, chunked launches), mirroring the
+reference's tests:
+  tests/test_cache_n54.cpp          cold -> disk -> in-process -> no-cache, bit-identical outputs
+  tests/test_cache_invalidation.cpp corrupt code object -> recompile and rewrite
+  tests/test_autotune_n54.cpp       sweep runs once, then the cache is hit
+  tests/test_bpb_sweep_n54.cpp      explicit pairs-per-workgroup sweep, same results
+  tests/test_nvrtc_multiN.cpp       every fixture size through the runtime path
+and NvrtcGeneigSolver::launch_chunked (src/nvrtc/nvrtc_solver.cpp:1078-1107).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(a, device):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def run(geneig, solver, A, B, device):
+    import torch
+
+    n = A.shape[-1]
+    W, U, info = geneig.empty_outputs(n, A.shape[0], device)
+    solver.launch(A if not isinstance(A, np.ndarray) else to_dev(A, device),
+                  B if not isinstance(B, np.ndarray) else to_dev(B, device), W, U, info)
+    torch.cuda.synchronize()
+    return W.cpu().numpy(), U.cpu().numpy(), info.cpu().numpy()
+
+
+@pytest.fixture
+def cache_dir(tmp_path, monkeypatch):
+    d = tmp_path / "hsaco"
+    monkeypatch.setenv("SUNNY_GENEIG_CACHE_DIR", str(d))
+    return d
+
+
+def test_cache_layers_bit_identical(geneig, device, cache_dir):
+    """compile -> disk -> in-process, outputs bit-identical (test_cache_n54.cpp:201-350)."""
+    A, B = geneig.gen_generic(40, 96, seed=17)
+    s1 = geneig.GeneigSolver(40)
+    assert s1.cache_layer_used() == "compile"
+    files = os.listdir(cache_dir)
+    assert any(f.endswith(".hsaco") for f in files), files
+    r1 = run(geneig, s1, A, B, device)
+    s2 = geneig.GeneigSolver(40)              # module still alive in-process
+    assert s2.cache_layer_used() == "in-process"
+    del s1, s2
+    import gc
+
+    gc.collect()
+    s3 = geneig.GeneigSolver(40)              # weak entry expired -> disk
+    assert s3.cache_layer_used() == "disk"
+    r3 = run(geneig, s3, A, B, device)
+    s4 = geneig.GeneigSolver(40, cache_mode=geneig.CACHE_NOCACHE)
+    assert s4.cache_layer_used() == "compile"
+    r4 = run(geneig, s4, A, B, device)
+    for r in (r3, r4):
+        assert r1[0].tobytes() == r[0].tobytes() and r1[1].tobytes() == r[1].tobytes()
+    print("acquire ms: disk %.1f" % s3.acquire_module_ms(), flush=True)
+
+
+def test_cache_invalidation_recompiles(geneig, device, cache_dir):
+    """Corrupt ELF magic -> discarded, recompiled, rewritten (test_cache_invalidation.cpp:238-312)."""
+    s = geneig.GeneigSolver(24)
+    del s
+    import gc
+
+    gc.collect()
+    paths = [os.path.join(cache_dir, f) for f in os.listdir(cache_dir) if f.endswith(".hsaco")]
+    assert paths
+    for path in paths:
+        with open(path, "r+b") as f:
+            f.write(b"XXXX")
+    s = geneig.GeneigSolver(24)
+    assert s.cache_layer_used() == "compile"
+    for path in paths:
+        with open(path, "rb") as f:
+            assert f.read(4) == b"\x7fELF"
+    # a truncated file is rejected too
+    del s
+    gc.collect()
+    for path in paths:
+        with open(path, "r+b") as f:
+            f.truncate(100)
+    s = geneig.GeneigSolver(24)
+    assert s.cache_layer_used() == "compile"
+    A, B = geneig.gen_generic(24, 8, seed=2)
+    W, U, info = run(geneig, s, A, B, device)
+    assert (info == 0).all()
+
+
+def test_autotune_then_cache_hit(geneig, oracle, device, cache_dir):
+    """Sweep on first construction, winner reused after (test_autotune_n54.cpp:308-346)."""
+    s1 = geneig.GeneigSolver(32, tuning_mode=geneig.TUNING_AUTOTUNE)
+    assert s1.was_autotuned() and s1.autotune_sweep_ran()
+    assert s1.batches_per_block() in (1, 2, 4, 8)
+    s2 = geneig.GeneigSolver(32, tuning_mode=geneig.TUNING_AUTOTUNE)
+    assert s2.was_autotuned() and not s2.autotune_sweep_ran()
+    assert s2.batches_per_block() == s1.batches_per_block()
+    assert any(f.startswith("autotune_32_") for f in os.listdir(cache_dir))
+    A, B = oracle.gen_generic(32, 16, seed=3)
+    W, X, info = run(geneig, s2, A, B, device)
+    Wo, Xo, _ = oracle.zhegv(A, B)
+    m = oracle.compare(A, B, W, X, Wo, Xo)
+    assert m["eig_rel_scaled"] < 1e-12 and m["phase_err"] < 1e-10
+
+
+@pytest.mark.parametrize("n,bpb", [(54, 1), (54, 2), (24, 4), (24, 8)])
+def test_bpb_sweep_same_results(geneig, oracle, device, cache_dir, n, bpb):
+    """Explicit pairs-per-workgroup, including a partial last workgroup
+    (the reference required padding; test_bpb_sweep_n54.cpp:234-338)."""
+    A, B = oracle.gen_generic(n, 13, seed=21)
+    s = geneig.GeneigSolver(n, batches_per_block_request=bpb)
+    assert s.batches_per_block() == bpb and s.block_dim() == (64, bpb, 1)
+    W, X, info = run(geneig, s, A, B, device)
+    assert (info == 0).all()
+    Wo, Xo, _ = oracle.zhegv(A, B)
+    m = oracle.compare(A, B, W, X, Wo, Xo)
+    assert m["eig_rel_scaled"] < 1e-12 and m["phase_err"] < 1e-10
+
+
+def test_bpb_that_does_not_fit_raises(geneig, device, cache_dir):
+    """An explicit pairs-per-workgroup beyond the LDS budget is refused, as the
+    reference refuses an explicit BPB that does not fit (nvrtc_solver.cpp:1021-1029)."""
+    with pytest.raises(geneig.GeneigError, match="LDS"):
+        geneig.GeneigSolver(54, batches_per_block_request=4)
+
+
+def test_runtime_matches_static_n54_bitwise(geneig, device, cache_dir):
+    """The hipRTC module and the hipcc build of the same source agree bit for bit."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 64, seed=4)
+    W1, U1, i1 = geneig.empty_outputs(54, 64, device)
+    geneig.full_n54_batched(A, B, W1, U1, i1)
+    W2, U2, i2 = run(geneig, geneig.GeneigSolver(54, batches_per_block_request=1), A, B, device)
+    torch.cuda.synchronize()
+    assert W1.cpu().numpy().tobytes() == W2.tobytes()
+    assert U1.cpu().numpy().tobytes() == U2.tobytes()
+
+
+def test_launch_chunked_streams(geneig, oracle, device, cache_dir):
+    """Chunks round-robined over streams give the same outputs as one launch."""
+    import torch
+
+    A, B = geneig.gen_generic(16, 100, seed=8)
+    s = geneig.GeneigSolver(16)
+    W0, U0, i0 = run(geneig, s, A, B, device)
+    W, U, info = geneig.empty_outputs(16, 100, device)
+    streams = geneig.make_streams(3)
+    torch.cuda.synchronize()
+    s.launch_chunked(A, B, W, U, info, total=100, chunk_size=17, streams=streams)
+    torch.cuda.synchronize()
+    assert np.array_equal(W.cpu().numpy(), W0) and np.array_equal(U.cpu().numpy(), U0)
+
+
+def test_workspace_chunking_small_budget(geneig, oracle, device):
+    """A batch larger than one workspace chunk goes through several K1-K2-K3 rounds."""
+    import subprocess
+    import sys
+
+    code = r'''
+import os, sys, numpy as np, torch
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import conftest, oracle as O
+g = conftest.load_package()
+A, B = g.gen_generic(54, 700, seed=12)
+W, U, info = g.empty_outputs(54, 700, "cuda")
+g.full_n54_batched(A, B, W, U, info); torch.cuda.synchronize()
+Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+Wo, Xo, io = O.zhegv(Ah, Bh)
+m = O.compare(Ah, Bh, W.cpu().numpy(), U.cpu().numpy(), Wo, Xo)
+assert int(info.abs().max()) == 0 and m["eig_rel_scaled"] < 1e-12 and m["phase_err"] < 1e-10, m
+print("ok", m["eig_rel_scaled"])
+''' % (os.path.join(os.path.dirname(GOLDEN)), os.path.join(os.path.dirname(os.path.dirname(GOLDEN)), "oracle"))
+    env = dict(os.environ, GENEIG_WORKSPACE_MB="16")   # 16 MiB -> chunks of ~236 pairs
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    print(out.stdout, out.stderr[-2000:], flush=True)
+    assert out.returncode == 0 and "ok" in out.stdout
