@@ -1,0 +1,47 @@
+/* geneig_mixed.h -- precision form of the batched n = 54 generalized
+ * Hermitian-definite eigensolve on gfx950.
+ *
+ * No counterpart in the reference: src/geneig_fixed.cu offers the FP64
+ * pipeline only (geneig_full_n54_batched_launch, src/geneig_fixed.cu:474).
+ * This entry point takes the same arguments and produces the same outputs
+ * (ascending eigenvalues W, B-orthonormal eigenvectors U, zhegv-numbered
+ * info), computed as:
+ *   1. FP64 Cholesky of B (B not positive definite: info = n + j, NaN outputs,
+ *      exactly as the FP64 path);
+ *   2. complex64 reduction C = L^-1 A L^-H, tridiagonalization, QL and
+ *      back-transform to an approximate X;
+ *   3. FP64 refinement of (lambda, X) against the original A and B on the FP64
+ *      matrix cores, until the largest correction is <= tol;
+ *   4. pairs whose refinement diverges, stalls after max_iter steps, or meets
+ *      coinciding eigenvalues are solved again by the FP64 pipeline.
+ *
+ * The launch synchronizes `stream` once, after step 3, to size step 4.
+ * max_iter <= 0 selects 4, tol <= 0 selects 1e-7 (the refined X then carries
+ * errors of order tol^2).  d_iters (nullable, device, batch_size ints)
+ * receives the refinement steps taken per pair (0: handed to step 4 before
+ * refining).  h_fallback (nullable, host) receives the number of pairs step 4
+ * solved again.  Returns a hipError_t value (0 = success).
+ */
+#ifndef GENEIG_MIXED_H
+#define GENEIG_MIXED_H
+
+#include <hip/hip_complex.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int geneig_mixed_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, double* d_W,
+                                    hipDoubleComplex* d_U, int* d_info, int batch_size, int max_iter,
+                                    double tol, int* d_iters, int* h_fallback, hipStream_t stream);
+
+/* Default refinement controls used when the launch is given max_iter <= 0 / tol <= 0. */
+int geneig_mixed_default_max_iter(void);
+double geneig_mixed_default_tol(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENEIG_MIXED_H */

@@ -53,6 +53,9 @@ def lib():
         "geneig_full_n54_launch": (I, [P, P, P, P, P, P]),
         "geneig_full_n54_batched_launch": (I, [P, P, P, P, P, I, P]),
         "geneig_finish_n54_launch": (I, [P, P, P, P, P, I, P]),
+        "geneig_mixed_n54_batched_launch": (I, [P, P, P, P, P, I, I, D, P, P, P]),
+        "geneig_mixed_default_max_iter": (I, []),
+        "geneig_mixed_default_tol": (D, []),
         "geneig_solver_create": (I, [P, I, I, I, I, I, I]),
         "geneig_solver_destroy": (None, [P]),
         "geneig_solver_launch": (I, [P, P, P, P, P, P, I, P]),
@@ -168,6 +171,27 @@ def full_n54_batched(H, S, W, U, info, batch=None, stream=None):
     _check_hip(lib().geneig_full_n54_batched_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
                                                     int(batch), _stream(stream)),
                "geneig_full_n54_batched_launch")
+
+
+def mixed_n54_batched(H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None):
+    """Precision form of the n = 54 batched solve (geneig_mixed_n54_batched_launch):
+    complex64 bulk, FP64 Cholesky and FP64 refinement, FP64 re-solve of the
+    pairs the refinement hands back.  Same outputs as ``full_n54_batched``.
+    ``iters`` (optional int32 device tensor of ``batch`` entries) receives the
+    refinement steps per pair.  Returns the number of pairs solved again in FP64."""
+    batch = H.shape[0] if batch is None else batch
+    _check_batch(54, batch, H, S, W, U, info)
+    if iters is not None:
+        _require(iters, _torch().int32, "iters")
+        if iters.numel() < batch:
+            raise GeneigError("iters holds %d entries, batch is %d" % (iters.numel(), batch))
+    fb = C.c_int(0)
+    _check_hip(lib().geneig_mixed_n54_batched_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
+                                                     int(max_iter), float(tol),
+                                                     _ptr(iters) if iters is not None else None,
+                                                     C.byref(fb), _stream(stream)),
+               "geneig_mixed_n54_batched_launch")
+    return fb.value
 
 
 def full_n54(H, S, W, U, info, stream=None):

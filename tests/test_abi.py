@@ -11,7 +11,7 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("geneig.h", "geneig_solver.h", "geneig_gen.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("geneig.h", "geneig_solver.h", "geneig_gen.h", "geneig_mixed.h")]
 
 
 def declared_functions():
@@ -28,7 +28,8 @@ def declared_functions():
 def test_headers_declare_entry_points():
     names = declared_functions()
     for must in ("geneig_full_n54_batched_launch", "chol_n54_launch", "geneig_finish_n54_launch",
-                 "geneig_solver_create", "geneig_solver_launch_chunked", "geneig_cache_path"):
+                 "geneig_solver_create", "geneig_solver_launch_chunked", "geneig_cache_path",
+                 "geneig_mixed_n54_batched_launch"):
         assert must in names
 
 
@@ -89,7 +90,8 @@ def test_cache_path_key(geneig, monkeypatch):
 
 
 def test_source_hash_is_fnv1a_of_device_source(geneig, oracle):
-    src = open(os.path.join(ROOT, "sunny-batched-geneig_amd", "csrc", "geneig_device.inc"), "rb").read()
+    csrc = os.path.join(ROOT, "sunny-batched-geneig_amd", "csrc")
+    src = b"".join(open(os.path.join(csrc, f), "rb").read() for f in ("geneig_device.inc", "geneig_refine.inc"))
     assert geneig.lib().geneig_source_hash() == oracle.fnv1a64(src)
     buf = C.create_string_buffer(b"foobar", 6)
     assert geneig.lib().geneig_fnv1a64(C.cast(buf, C.c_void_p), 6, 0) == 0x85944171f73967e8

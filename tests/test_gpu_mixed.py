@@ -1,0 +1,126 @@
+"""GPU parity tests of the precision form (geneig_mixed_n54_batched_launch):
+complex64 bulk, FP64 Cholesky, FP64 refinement on the matrix cores, FP64
+re-solve of the pairs the refinement hands back.  Its outputs must meet the
+bar of the FP64 path against the oracle (restated zhegv): eigenvalue error
+<= 1e-12 relative to the largest |lambda|, scale-invariant residual and
+B-orthogonality <= 50 n eps, eigenvectors equal up to phase.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import check_pairs, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed(geneig, A, B, device, **kw):
+    import torch
+
+    batch = A.shape[0]
+    dA = A if isinstance(A, torch.Tensor) else to_dev(A, device)
+    dB = B if isinstance(B, torch.Tensor) else to_dev(B, device)
+    W, U, info = geneig.empty_outputs(54, batch, device)
+    iters = torch.full((batch,), -1, dtype=torch.int32, device=device)
+    fb = geneig.mixed_n54_batched(dA, dB, W, U, info, iters=iters, **kw)
+    torch.cuda.synchronize()
+    return to_host(W), to_host(U), to_host(info), to_host(iters), fb
+
+
+def hist(iters):
+    v, c = np.unique(iters, return_counts=True)
+    return {int(a): int(b) for a, b in zip(v, c)}
+
+
+def test_mixed_spinwave_vs_oracle(geneig, oracle, device):
+    A, B = geneig.gen_spinwave(27, 1024, seed=1, first_q=5000)
+    W, X, info, iters, fb = mixed(geneig, A, B, device)
+    Ah, Bh = to_host(A), to_host(B)
+    print("spin-wave n=54: fallback %d of 1024, refinement steps %s" % (fb, hist(iters)), flush=True)
+    assert (info == 0).all()
+    assert fb <= 1024 // 20
+    Wo, Xo, io = oracle.zhegv(Ah, Bh)
+    check_pairs(oracle, Ah, Bh, W, X, Wo, Xo, 54, "mixed spin-wave 1024")
+
+
+def test_mixed_generic_vs_oracle(geneig, oracle, device):
+    A, B = oracle.gen_generic(54, 256, seed=77)
+    W, X, info, iters, fb = mixed(geneig, A, B, device)
+    print("generic n=54: fallback %d of 256, refinement steps %s" % (fb, hist(iters)), flush=True)
+    assert (info == 0).all() and fb == 0
+    Wo, Xo, io = oracle.zhegv(A, B)
+    check_pairs(oracle, A, B, W, X, Wo, Xo, 54, "mixed generic 256")
+
+
+def test_mixed_fallback_matches_fp64_bitwise(geneig, oracle, device):
+    """With a tolerance no refinement can meet every pair is re-solved by the
+    FP64 pipeline over the pair list: outputs bit-identical to the FP64 path."""
+    import torch
+
+    A, B = oracle.gen_generic(54, 37, seed=5)
+    W, X, info, iters, fb = mixed(geneig, A, B, device, max_iter=1, tol=1e-300)
+    assert fb == 37 and (iters == 1).all()
+    dA, dB = to_dev(A, device), to_dev(B, device)
+    W2, U2, i2 = geneig.empty_outputs(54, 37, device)
+    geneig.full_n54_batched(dA, dB, W2, U2, i2)
+    torch.cuda.synchronize()
+    assert W.tobytes() == to_host(W2).tobytes()
+    assert X.tobytes() == to_host(U2).tobytes()
+    assert (info == to_host(i2)).all()
+
+
+def test_mixed_not_positive_definite(geneig, oracle, device):
+    """B with a negative pivot: info = n + j and NaN outputs, as the FP64 path."""
+    A, B = oracle.gen_generic(54, 3, seed=11)
+    B = B.copy()
+    B[1, 5, 5] = -1.0
+    W, X, info, iters, fb = mixed(geneig, A, B, device)
+    Wo, Xo, io = oracle.zhegv(A, B)
+    assert info[0] == 0 and info[2] == 0 and info[1] == io[1] > 54
+    assert np.isnan(W[1]).all() and np.isnan(X[1]).all()
+    assert fb == 0
+    keep = [0, 2]
+    check_pairs(oracle, A[keep], B[keep], W[keep], X[keep], Wo[keep], Xo[keep], 54, "mixed non-PD neighbours")
+
+
+@pytest.mark.parametrize("cond", [1e2, 1e4, 1e6, 1e8])
+def test_mixed_condition_sweep(geneig, oracle, device, cond):
+    """cond(B) from 1e2 to 1e8: the refinement converges or the pair is re-solved
+    in FP64.  Every metric against the oracle meets the FP64 bar or, where
+    cond(B) puts the FP64 path itself above that bar, stays within twice the
+    FP64 path's figure on the same pairs."""
+    import torch
+
+    from test_gpu_parity import EPS, TOL_EIG, TOL_PHASE
+
+    A, B = oracle.gen_generic_cond(54, 64, seed=3, cond=cond)
+    W, X, info, iters, fb = mixed(geneig, A, B, device)
+    assert (info == 0).all()
+    Wo, Xo, io = oracle.zhegv(A, B)
+    W2, U2, i2 = geneig.empty_outputs(54, 64, device)
+    geneig.full_n54_batched(to_dev(A, device), to_dev(B, device), W2, U2, i2)
+    torch.cuda.synchronize()
+    mp = oracle.compare(A, B, to_host(W2), to_host(U2), Wo, Xo)
+    mm = oracle.compare(A, B, W, X, Wo, Xo)
+    print("cond %.0e: fallback %d of 64, refinement steps %s" % (cond, fb, hist(iters)), flush=True)
+    bars = {"eig_rel_scaled": TOL_EIG, "phase_err": TOL_PHASE, "resid": 50 * 54 * EPS, "orth": 50 * 54 * EPS}
+    for k, bar in bars.items():
+        print("  %-15s mixed %.2e  fp64 path %.2e  bar %.1e" % (k, mm[k], mp[k], bar), flush=True)
+        assert mm[k] <= max(bar, 2.0 * mp[k]), k
+    if cond <= 1e4:
+        assert fb == 0
+
+
+@pytest.mark.parametrize("batch", [0, 1, 7])
+def test_mixed_batch_edges(geneig, oracle, device, batch):
+    import torch
+
+    A, B = oracle.gen_generic(54, max(batch, 1), seed=3)
+    W, U, info = geneig.empty_outputs(54, max(batch, 1), device)
+    W.fill_(-7.0)
+    fb = geneig.mixed_n54_batched(to_dev(A, device), to_dev(B, device), W, U, info, batch=batch)
+    torch.cuda.synchronize()
+    if batch == 0:
+        assert fb == 0 and (to_host(W) == -7.0).all()
+        return
+    Wo, Xo, io = oracle.zhegv(A, B)
+    check_pairs(oracle, A, B, to_host(W), to_host(U), Wo, Xo, 54, "mixed batch=%d" % batch)
