@@ -127,6 +127,77 @@ def cpu_baseline(A, B, target_s=12.0):
     return out
 
 
+MIXED_SIZES = [8, 16, 24, 32, 48, 54, 64, 96]
+MIXED_COUNTS = [72304, 46998, 30548, 19856, 12907, 8389, 5453, 3545]   # 0.35*0.65^k, 200,000 pairs
+
+
+def bench_mixed_sizes(args, geneig, torch, dev):
+    """Mixed-size batch: generic pairs (A = (M+M^H)/(2 sqrt n), B = G G^H/n + I)
+    binned by n, solved by one graph launch (one branch per size, per-size
+    pairs-per-workgroup from the autotuner) against one launch per size."""
+    t0 = time.perf_counter()
+    m = geneig.GeneigMulti(MIXED_SIZES, tuning_mode=geneig.TUNING_AUTOTUNE)
+    log("solvers ready in %.1f s; pairs per workgroup %s"
+        % (time.perf_counter() - t0, [m.batches_per_block(k) for k in range(len(MIXED_SIZES))]))
+    bins = []
+    for k, (n, c) in enumerate(zip(MIXED_SIZES, MIXED_COUNTS)):
+        A, B = geneig.gen_generic(n, c, seed=args.seed + k)
+        W, U, info = geneig.empty_outputs(n, c, dev)
+        bins.append((A, B, W, U, info))
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    total = sum(MIXED_COUNTS)
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        t_start = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(stream)
+            step()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        return wall, float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs])) * 1e-3
+
+    wall_s, launch_s = timed(lambda: m.launch_serial(bins, stream=stream))
+    wall_g, launch_g = timed(lambda: m.launch(bins, stream=stream))
+    log("serial %.2f ms/step, graph %.2f ms/step (captures %d)"
+        % (wall_s * 1e3 / args.steps, wall_g * 1e3 / args.steps, m.captures()))
+    acc = {}
+    for k, n in enumerate(MIXED_SIZES):
+        A, B, W, U, info = bins[k]
+        sel = torch.arange(0, A.shape[0], max(1, A.shape[0] // 16), device=dev)[:16]
+        a = accuracy(A.index_select(0, sel).cpu().numpy(), B.index_select(0, sel).cpu().numpy(),
+                     W.index_select(0, sel).cpu().numpy(), U.index_select(0, sel).cpu().numpy())
+        acc[str(n)] = {"max_eig_err_over_max_abs_eig": a["max_eig_err_over_max_abs_eig"],
+                       "max_resid_over_n_eps": a["max_resid_over_n_eps"],
+                       "max_orth_over_n_eps": a["max_orth_over_n_eps"], "info_nonzero": int((info != 0).sum().item())}
+    flops = sum(c * flops_per_pair(n) for n, c in zip(MIXED_SIZES, MIXED_COUNTS))
+    nbytes = sum(c * bytes_per_pair(n) for n, c in zip(MIXED_SIZES, MIXED_COUNTS))
+    line = {
+        "metric": "mixed-size geneig solves/s (n in {8,16,24,32,48,54,64,96}, 200,000 pairs), one graph launch",
+        "value": total / (wall_g / args.steps), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall_g * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (device-generated generic pairs, seed %d + bin)" % args.seed,
+        "config": {"workload": "generic pairs binned by n", "sizes": MIXED_SIZES, "counts": MIXED_COUNTS,
+                   "pairs_per_workgroup": [m.batches_per_block(k) for k in range(len(MIXED_SIZES))],
+                   "parallelism": "none (1 GPU)"},
+        "serial": {"value": total / (wall_s / args.steps), "ms_per_step": wall_s * 1e3 / args.steps},
+        "speedup_vs_serial": wall_s / wall_g,
+        "accuracy_sampled_16_per_size": acc,
+        "roofline": {"bound": "mfma", "achieved": flops / launch_g / 1e12, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": flops / launch_g / FP64_PEAK, "traffic": None,
+                     "kernel": "one graph launch (all bins)", "avg_launch_ms": launch_g * 1e3,
+                     "hbm_frac": nbytes / launch_g / HBM_PEAK},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +208,9 @@ def main():
     ap.add_argument("--check", type=int, default=256, help="pairs checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["spinwave54", "mixed-sizes"], default="spinwave54",
+                    help="spinwave54: the headline metric; mixed-sizes: 200,000 generic pairs over "
+                         "n in {8,...,96}, one graph launch vs one launch per size")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -162,6 +236,9 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_per_dispatch": nbytes}), flush=True)
         return
     geneig = load_package()
+    if args.workload == "mixed-sizes":
+        bench_mixed_sizes(args, geneig, torch, dev)
+        return
     n, nsub, batch = 54, 27, args.batch
     grid = (64, 64, 32)
 

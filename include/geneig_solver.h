@@ -54,6 +54,12 @@ int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void*
                                  double* d_W, void* d_U, int* d_info, int total_matrices,
                                  int chunk_size, int num_streams, const hipStream_t* streams);
 
+/* Allocates, ahead of time, the device workspace a launch of batch_size
+ * pairs on `stream` will use, so that the launch itself allocates nothing
+ * and can be captured into a graph.  No counterpart in the reference (whose
+ * kernel used shared memory only). */
+int geneig_solver_reserve(geneig_solver_t s, int batch_size, hipStream_t stream);
+
 /* Accessors (nvrtc_solver.hpp:120-140). */
 int geneig_solver_matrix_size(geneig_solver_t s);
 int geneig_solver_batches_per_block(geneig_solver_t s);

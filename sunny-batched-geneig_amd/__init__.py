@@ -70,6 +70,15 @@ def lib():
         "geneig_solver_cache_layer_used": (C.c_char_p, [P]),
         "geneig_solver_acquire_module_ms": (D, [P]),
         "geneig_solver_variant": (I, [P]),
+        "geneig_solver_reserve": (I, [P, I, P]),
+        "geneig_multi_create": (I, [P, I, P, I, I, I]),
+        "geneig_multi_destroy": (None, [P]),
+        "geneig_multi_launch": (I, [P, P, P, P, P, P, P, P]),
+        "geneig_multi_launch_serial": (I, [P, P, P, P, P, P, P, P]),
+        "geneig_multi_num_sizes": (I, [P]),
+        "geneig_multi_solver": (P, [P, I]),
+        "geneig_multi_captures": (C.c_longlong, [P]),
+        "geneig_multi_replays": (C.c_longlong, [P]),
         "geneig_cache_dir": (C.c_size_t, [C.c_char_p, C.c_size_t]),
         "geneig_cache_path": (C.c_size_t, [I, I, C.c_char_p, I, I, C.c_char_p, C.c_size_t]),
         "geneig_rtc_compile": (I, [I, I, I, C.c_char_p, P, C.c_size_t, P]),
@@ -291,6 +300,10 @@ class GeneigSolver:
     def variant(self):
         return lib().geneig_solver_variant(self._h)
 
+    def reserve(self, batch, stream=None):
+        """Pre-allocate the workspace a launch of `batch` pairs on `stream` uses."""
+        _check(lib().geneig_solver_reserve(self._h, int(batch), _stream(stream)), "GeneigSolver.reserve")
+
     @staticmethod
     def cache_dir():
         buf = C.create_string_buffer(4096)
@@ -303,6 +316,59 @@ class GeneigSolver:
         lib().geneig_cache_path(int(n), int(bpb), arch.encode(), int(block_dim_x), int(variant), buf, 4096)
         return buf.value.decode()
 
+
+class GeneigMulti:
+    """Mixed-size batches (geneig_multi_*): one solver per size; ``launch`` solves
+    every bin in one graph launch (one branch per size, captured once and
+    replayed), ``launch_serial`` one solve launch per size."""
+
+    def __init__(self, sizes, device_id=0, tuning_mode=TUNING_AUTOTUNE, cache_mode=CACHE_AUTO):
+        self.sizes = [int(n) for n in sizes]
+        self._h = C.c_void_p()
+        arr = (C.c_int * len(self.sizes))(*self.sizes)
+        _check(lib().geneig_multi_create(C.byref(self._h), len(self.sizes), C.cast(arr, C.c_void_p),
+                                         int(device_id), int(tuning_mode), int(cache_mode)),
+               "GeneigMulti(%s)" % self.sizes)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.geneig_multi_destroy(h)
+            self._h = None
+
+    def _args(self, bins):
+        """bins: one (A, B, W, U, info) tuple of device tensors per size (None or
+        an empty A for a size with no pairs)."""
+        K = len(self.sizes)
+        if len(bins) != K:
+            raise GeneigError("need %d bins, got %d" % (K, len(bins)))
+        counts = (C.c_int * K)()
+        ptrs = [(C.c_void_p * K)() for _ in range(5)]
+        for k, b in enumerate(bins):
+            if b is None or b[0].shape[0] == 0:
+                continue
+            A, B, W, U, info = b
+            _check_batch(self.sizes[k], A.shape[0], A, B, W, U, info)
+            counts[k] = A.shape[0]
+            for j, t in enumerate((A, B, W, U, info)):
+                ptrs[j][k] = t.data_ptr()
+        return [C.cast(counts, C.c_void_p)] + [C.cast(p, C.c_void_p) for p in ptrs]
+
+    def launch(self, bins, stream=None):
+        _check(lib().geneig_multi_launch(self._h, *self._args(bins), _stream(stream)), "GeneigMulti.launch")
+
+    def launch_serial(self, bins, stream=None):
+        _check(lib().geneig_multi_launch_serial(self._h, *self._args(bins), _stream(stream)),
+               "GeneigMulti.launch_serial")
+
+    def batches_per_block(self, k):
+        return lib().geneig_solver_batches_per_block(lib().geneig_multi_solver(self._h, int(k)))
+
+    def captures(self):
+        return lib().geneig_multi_captures(self._h)
+
+    def replays(self):
+        return lib().geneig_multi_replays(self._h)
 
 def rtc_compile(n, bpb=1, variant=0, arch="gfx950"):
     """Device-free hipRTC compile of the per-n kernel; returns the code object bytes."""

@@ -33,14 +33,15 @@
 #include <vector>
 
 #include "../../include/geneig_solver.h"
+#include "geneig_err.hpp"
 #include "geneig_ws.hpp"
 #include "geneig_device_src.inc"  // kGeneigDeviceSource: geneig_device.inc as a string
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int geneig_host::fail(int code, const char* fmt, ...) {
     char buf[2048];
     va_list ap;
     va_start(ap, fmt);
@@ -51,12 +52,10 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t _e = (expr);                                                           \
-        if (_e != hipSuccess)                                                             \
-            return fail(GENEIG_E_HIP, "%s failed: %s", #expr, hipGetErrorString(_e));    \
-    } while (0)
+namespace {
+
+using geneig_host::fail;
+#define HIP_TRY(expr) GENEIG_HIP_TRY(expr)
 
 // ---- content hashes (FNV-1a 64, the reference's cache fingerprint,
 //      src/nvrtc/nvrtc_solver.cpp:321-333) ------------------------------------
@@ -654,6 +653,15 @@ int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void*
                                       streams[ci % nstreams]);
         if (rc) return rc;
     }
+    return GENEIG_E_OK;
+}
+
+int geneig_solver_reserve(geneig_solver_t s, int batch_size, hipStream_t stream) {
+    if (!s) return fail(GENEIG_E_ARG, "null solver");
+    if (batch_size <= 0) return GENEIG_E_OK;
+    const int chunk = geneig_host::chunk_pairs(batch_size, s->module->ws_pair_bytes);
+    void* ws = nullptr;
+    HIP_TRY(geneig_host::workspace(stream, (size_t)chunk * s->module->ws_pair_bytes, &ws));
     return GENEIG_E_OK;
 }
 
