@@ -28,6 +28,8 @@
 #include <hip/hip_complex.h>
 #include <hip/hip_runtime_api.h>
 
+#include "geneig_solver.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -35,6 +37,16 @@ extern "C" {
 int geneig_mixed_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, double* d_W,
                                     hipDoubleComplex* d_U, int* d_info, int batch_size, int max_iter,
                                     double tol, int* d_iters, int* h_fallback, hipStream_t stream);
+
+/* Precision form for any n (1..128): a runtime solver (geneig_solver.h) whose
+ * hipRTC module holds the precision-form kernels specialized for n (cached
+ * like the FP64 module, variant 1 in the cache key).  geneig_solver_launch on
+ * it runs the precision form with the default controls; this launch exposes
+ * them.  Destroy with geneig_solver_destroy. */
+int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int cache_mode);
+int geneig_mixed_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W, void* d_U,
+                               int* d_info, int batch_size, int max_iter, double tol, int* d_iters,
+                               int* h_fallback, hipStream_t stream);
 
 /* Default refinement controls used when the launch is given max_iter <= 0 / tol <= 0. */
 int geneig_mixed_default_max_iter(void);

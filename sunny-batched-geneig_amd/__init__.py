@@ -56,6 +56,8 @@ def lib():
         "geneig_mixed_n54_batched_launch": (I, [P, P, P, P, P, I, I, D, P, P, P]),
         "geneig_mixed_default_max_iter": (I, []),
         "geneig_mixed_default_tol": (D, []),
+        "geneig_mixed_solver_create": (I, [P, I, I, I]),
+        "geneig_mixed_solver_launch": (I, [P, P, P, P, P, P, I, I, D, P, P, P]),
         "geneig_solver_create": (I, [P, I, I, I, I, I, I]),
         "geneig_solver_destroy": (None, [P]),
         "geneig_solver_launch": (I, [P, P, P, P, P, P, I, P]),
@@ -315,6 +317,31 @@ class GeneigSolver:
         buf = C.create_string_buffer(4096)
         lib().geneig_cache_path(int(n), int(bpb), arch.encode(), int(block_dim_x), int(variant), buf, 4096)
         return buf.value.decode()
+
+
+class GeneigMixedSolver(GeneigSolver):
+    """Precision form for any n (geneig_mixed_solver_*): complex64 bulk, FP64
+    Cholesky and refinement, FP64 re-solve of the pairs the refinement hands
+    back.  ``launch`` (inherited) runs it with the default controls;
+    ``launch_mixed`` exposes them and returns the number of re-solved pairs."""
+
+    def __init__(self, n, device_id=0, cache_mode=CACHE_AUTO):
+        self._h = C.c_void_p()
+        _check(lib().geneig_mixed_solver_create(C.byref(self._h), int(n), int(device_id), int(cache_mode)),
+               "GeneigMixedSolver(n=%d)" % n)
+
+    def launch_mixed(self, H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None):
+        batch = H.shape[0] if batch is None else batch
+        _check_batch(self.matrix_size(), batch, H, S, W, U, info)
+        if iters is not None:
+            _require(iters, _torch().int32, "iters")
+            if iters.numel() < batch:
+                raise GeneigError("iters holds %d entries, batch is %d" % (iters.numel(), batch))
+        fb = C.c_int(0)
+        _check(lib().geneig_mixed_solver_launch(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
+                                                int(max_iter), float(tol), _ptr(iters), C.byref(fb), _stream(stream)),
+               "GeneigMixedSolver.launch_mixed")
+        return fb.value
 
 
 class GeneigMulti:

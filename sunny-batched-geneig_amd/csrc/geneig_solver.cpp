@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "../../include/geneig_solver.h"
+#include "../../include/geneig_mixed.h"
 #include "geneig_err.hpp"
 #include "geneig_ws.hpp"
 #include "geneig_device_src.inc"  // kGeneigDeviceSource: geneig_device.inc as a string
@@ -157,7 +158,7 @@ bool write_atomic(const std::string& path, const void* data, size_t size) {
 int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vector<char>& code) {
     if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
     if (bpb < 1 || bpb > 16) return fail(GENEIG_E_ARG, "pairs per workgroup %d outside 1..16", bpb);
-    if (variant != 0) return fail(GENEIG_E_ARG, "eigen-algorithm variant %d is not built", variant);
+    if (variant != 0 && variant != 1) return fail(GENEIG_E_ARG, "variant %d is not built", variant);
     hiprtcProgram prog = nullptr;
     if (hiprtcCreateProgram(&prog, kGeneigDeviceSource, "geneig_device.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
@@ -196,6 +197,12 @@ struct Module {
     unsigned lds_total = 0;        // K1 / K3 dynamic + static LDS bytes per pair
     unsigned k2_threads = 0;
     unsigned long long ws_pair_bytes = 0;
+    // precision form (variant 1): FP64 Cholesky, complex64 K1/K2/K3, FP64
+    // refinement, FP64 K1/K2/K3 over a pair list, counter clear
+    hipFunction_t fc = nullptr, f1f = nullptr, f2f = nullptr, f3f = nullptr, fr = nullptr;
+    hipFunction_t f1l = nullptr, f2l = nullptr, f3l = nullptr, fz = nullptr;
+    unsigned f_lds_per_pair = 0, f_k2_threads = 0, refine_lds = 0, refine_threads = 0;
+    unsigned long long f_ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
     Module() = default;
     Module(const Module&) = delete;
@@ -212,9 +219,21 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     m->bpb = bpb;
     m->variant = variant;
     HIP_TRY(hipModuleLoadData(&m->mod, code.data()));
-    HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_k1"));
-    HIP_TRY(hipModuleGetFunction(&m->fn2, m->mod, "geneig_k2"));
-    HIP_TRY(hipModuleGetFunction(&m->fn3, m->mod, "geneig_k3"));
+    if (variant == 1) {
+        HIP_TRY(hipModuleGetFunction(&m->fc, m->mod, "geneig_chol64"));
+        HIP_TRY(hipModuleGetFunction(&m->f1f, m->mod, "geneig_k1f"));
+        HIP_TRY(hipModuleGetFunction(&m->f2f, m->mod, "geneig_k2f"));
+        HIP_TRY(hipModuleGetFunction(&m->f3f, m->mod, "geneig_k3f"));
+        HIP_TRY(hipModuleGetFunction(&m->fr, m->mod, "geneig_refine"));
+        HIP_TRY(hipModuleGetFunction(&m->f1l, m->mod, "geneig_k1l"));
+        HIP_TRY(hipModuleGetFunction(&m->f2l, m->mod, "geneig_k2l"));
+        HIP_TRY(hipModuleGetFunction(&m->f3l, m->mod, "geneig_k3l"));
+        HIP_TRY(hipModuleGetFunction(&m->fz, m->mod, "geneig_zero"));
+    } else {
+        HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_k1"));
+        HIP_TRY(hipModuleGetFunction(&m->fn2, m->mod, "geneig_k2"));
+        HIP_TRY(hipModuleGetFunction(&m->fn3, m->mod, "geneig_k3"));
+    }
     auto read_global = [&](const char* name, void* dst, size_t want) -> int {
         hipDeviceptr_t ptr = nullptr;
         size_t size = 0;
@@ -227,6 +246,13 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     if (!rc) rc = read_global("geneig_lds_total_per_pair", &m->lds_total, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_k2_threads", &m->k2_threads, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
+    if (variant == 1) {
+        if (!rc) rc = read_global("geneig_f_lds_bytes_per_pair", &m->f_lds_per_pair, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_f_k2_threads", &m->f_k2_threads, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_f_ws_bytes_per_pair", &m->f_ws_pair_bytes, sizeof(unsigned long long));
+        if (!rc) rc = read_global("geneig_refine_lds_bytes", &m->refine_lds, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_refine_threads", &m->refine_threads, sizeof(unsigned));
+    }
     if (rc) return rc;
     out = m;
     return GENEIG_E_OK;
@@ -409,7 +435,79 @@ struct geneig_solver {
         return GENEIG_E_OK;
     }
 
+    // Precision form over the batch (the sequence of geneig_mixed.hip with
+    // the module's kernels): FP64 Cholesky, complex64 K1/K2/K3 per chunk,
+    // FP64 refinement, then the FP64 pipeline over the pairs it lists.
+    // Synchronizes `stream` once, to read the fallback count.
+    int launch_mixed(const Module& m, const void* H, const void* S, double* W, void* U, int* info, int batch,
+                     int maxit, double tol, int* iters, int* h_fallback, hipStream_t stream) {
+        if (h_fallback) *h_fallback = 0;
+        if (batch <= 0) return GENEIG_E_OK;
+        if (maxit <= 0) maxit = 4;
+        if (!(tol > 0.0)) tol = 1e-7;
+        const size_t list_bytes = ((sizeof(int) * ((size_t)batch + 1) + 255) / 256) * 256;
+        const int chunk_f = geneig_host::chunk_pairs(batch, m.f_ws_pair_bytes);
+        const int chunk_d = geneig_host::chunk_pairs(batch, m.ws_pair_bytes);
+        const size_t solver_bytes = std::max((size_t)((size_t)chunk_f * m.f_ws_pair_bytes), (size_t)((size_t)chunk_d * m.ws_pair_bytes));
+        void* ws = nullptr;
+        HIP_TRY(geneig_host::workspace(stream, list_bytes + solver_bytes, &ws));
+        int* fb_count = static_cast<int*>(ws);
+        int* fb_list = fb_count + 1;
+        void* solver = static_cast<char*>(ws) + list_bytes;
+        const size_t mat = (size_t)n * n * 16;
+        void* az[] = {&fb_count};
+        HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 1, 1, 1, 0, stream, az, nullptr));
+        void* ac[] = {&S, &W, &U, &info, &batch};
+        HIP_TRY(hipModuleLaunchKernel(m.fc, (unsigned)batch, 1, 1, 64, 1, 1, m.lds_per_pair, stream, ac, nullptr));
+        for (int off = 0; off < batch; off += chunk_f) {
+            int P = std::min(chunk_f, batch - off);
+            const void* h = static_cast<const char*>(H) + mat * off;
+            const void* sp = static_cast<const char*>(S) + mat * off;
+            double* w = W + (size_t)n * off;
+            void* u = static_cast<char*>(U) + mat * off;
+            int* inf = info + off;
+            void* a1[] = {&h, &sp, &w, &u, &inf, &solver, &P};
+            HIP_TRY(hipModuleLaunchKernel(m.f1f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a1, nullptr));
+            void* a2[] = {&inf, &solver, &P};
+            HIP_TRY(hipModuleLaunchKernel(m.f2f, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
+            void* a3[] = {&sp, &w, &u, &inf, &solver, &P};
+            HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a3, nullptr));
+        }
+        void* ar[] = {&H, &S, &W, &U, &info, &fb_list, &fb_count, &iters, &batch, &maxit, &tol};
+        HIP_TRY(hipModuleLaunchKernel(m.fr, (unsigned)batch, 1, 1, m.refine_threads, 1, 1, m.refine_lds, stream, ar,
+                                      nullptr));
+        int count = 0;
+        HIP_TRY(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (count < 0 || count > batch) return fail(GENEIG_E_HIP, "fallback count %d out of range", count);
+        if (h_fallback) *h_fallback = count;
+        for (int off = 0; off < count; off += chunk_d) {
+            int P = std::min(chunk_d, count - off);
+            const int* lst = fb_list + off;
+            void* a1[] = {&H, &S, &W, &U, &info, &solver, &P, &lst};
+            HIP_TRY(hipModuleLaunchKernel(m.f1l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a1, nullptr));
+            void* a2[] = {&info, &solver, &P, &lst};
+            HIP_TRY(hipModuleLaunchKernel(m.f2l, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            void* a3[] = {&S, &W, &U, &info, &solver, &P, &lst};
+            HIP_TRY(hipModuleLaunchKernel(m.f3l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a3, nullptr));
+        }
+        return GENEIG_E_OK;
+    }
+
     void set_attrs(const Module& m, int b) {
+        if (m.variant == 1) {
+            auto set = [](hipFunction_t f, unsigned bytes) {
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)bytes);
+            };
+            set(m.fc, m.lds_per_pair);
+            set(m.f1l, m.lds_per_pair);
+            set(m.f3l, m.lds_per_pair);
+            set(m.f1f, m.f_lds_per_pair);
+            set(m.f3f, m.f_lds_per_pair);
+            set(m.fr, m.refine_lds);
+            return;
+        }
         const int bytes = (int)(m.lds_per_pair * (unsigned)b);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn3), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -634,7 +732,51 @@ void geneig_solver_destroy(geneig_solver_t s) { delete s; }
 int geneig_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W, void* d_U,
                          int* d_info, int batch_size, hipStream_t stream) {
     if (!s) return fail(GENEIG_E_ARG, "null solver");
+    if (s->variant == 1)
+        return s->launch_mixed(*s->module, d_H, d_S, d_W, d_U, d_info, batch_size, 0, 0.0, nullptr, nullptr, stream);
     return s->launch_raw(*s->module, s->bpb, d_H, d_S, d_W, d_U, d_info, batch_size, stream);
+}
+
+int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int cache_mode) {
+    if (!out) return fail(GENEIG_E_ARG, "null output handle");
+    *out = nullptr;
+    if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
+    if (cache_mode < 0 || cache_mode > 2) return fail(GENEIG_E_ARG, "bad cache mode %d", cache_mode);
+    std::unique_ptr<geneig_solver> s(new geneig_solver);
+    s->n = n;
+    s->device = device_id;
+    HIP_TRY(hipSetDevice(device_id));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device_id));
+    s->arch = prop.gcnArchName;
+    int v = 0;
+    HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device_id));
+    s->lds_per_block = (size_t)v;
+    const auto t0 = std::chrono::steady_clock::now();
+    const char* lay = "compile";
+    const int rc = s->acquire(1, 1, cache_mode, s->module, &lay);
+    if (rc) return rc;
+    s->acquire_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->layer = lay;
+    s->bpb = 1;
+    s->variant = 1;
+    const Module& m = *s->module;
+    s->shared_bytes = std::max(m.lds_per_pair, std::max(m.f_lds_per_pair, m.refine_lds));
+    if (s->shared_bytes > s->lds_per_block)
+        return fail(GENEIG_E_FIT, "n=%d: precision form needs %u B LDS per workgroup > %zu", n, s->shared_bytes,
+                    s->lds_per_block);
+    s->set_attrs(m, 1);
+    *out = s.release();
+    return GENEIG_E_OK;
+}
+
+int geneig_mixed_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W, void* d_U,
+                               int* d_info, int batch_size, int max_iter, double tol, int* d_iters,
+                               int* h_fallback, hipStream_t stream) {
+    if (!s) return fail(GENEIG_E_ARG, "null solver");
+    if (s->variant != 1) return fail(GENEIG_E_ARG, "solver n=%d was not created for the precision form", s->n);
+    return s->launch_mixed(*s->module, d_H, d_S, d_W, d_U, d_info, batch_size, max_iter, tol, d_iters, h_fallback,
+                           stream);
 }
 
 int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W,
@@ -659,9 +801,13 @@ int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void*
 int geneig_solver_reserve(geneig_solver_t s, int batch_size, hipStream_t stream) {
     if (!s) return fail(GENEIG_E_ARG, "null solver");
     if (batch_size <= 0) return GENEIG_E_OK;
-    const int chunk = geneig_host::chunk_pairs(batch_size, s->module->ws_pair_bytes);
+    const Module& m = *s->module;
+    size_t bytes = (size_t)geneig_host::chunk_pairs(batch_size, m.ws_pair_bytes) * m.ws_pair_bytes;
+    if (s->variant == 1)
+        bytes = ((sizeof(int) * ((size_t)batch_size + 1) + 255) / 256) * 256 +
+                std::max(bytes, (size_t)((size_t)geneig_host::chunk_pairs(batch_size, m.f_ws_pair_bytes) * m.f_ws_pair_bytes));
     void* ws = nullptr;
-    HIP_TRY(geneig_host::workspace(stream, (size_t)chunk * s->module->ws_pair_bytes, &ws));
+    HIP_TRY(geneig_host::workspace(stream, bytes, &ws));
     return GENEIG_E_OK;
 }
 

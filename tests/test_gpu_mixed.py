@@ -124,3 +124,53 @@ def test_mixed_batch_edges(geneig, oracle, device, batch):
         return
     Wo, Xo, io = oracle.zhegv(A, B)
     check_pairs(oracle, A, B, to_host(W), to_host(U), Wo, Xo, 54, "mixed batch=%d" % batch)
+
+
+def test_runtime_precision_n54_matches_static_bitwise(geneig, device):
+    """The hipRTC precision module for n = 54 and the hipcc build of the same
+    source agree bit for bit."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 96, seed=12)
+    W1, U1, i1 = geneig.empty_outputs(54, 96, device)
+    geneig.mixed_n54_batched(A, B, W1, U1, i1)
+    s = geneig.GeneigMixedSolver(54, cache_mode=geneig.CACHE_NODISK)
+    W2, U2, i2 = geneig.empty_outputs(54, 96, device)
+    fb = s.launch_mixed(A, B, W2, U2, i2)
+    torch.cuda.synchronize()
+    assert fb == 0 and s.variant() == 1
+    assert W1.cpu().numpy().tobytes() == W2.cpu().numpy().tobytes()
+    assert U1.cpu().numpy().tobytes() == U2.cpu().numpy().tobytes()
+
+
+@pytest.mark.parametrize("n,cond", [(96, 0), (96, 1e4), (96, 1e8), (16, 0), (33, 0), (64, 1e2)])
+def test_runtime_precision_vs_oracle(geneig, oracle, device, n, cond):
+    """Precision form through the runtime solver at other n, generic pairs and
+    controlled cond(B); same bar as the n = 54 sweep."""
+    import torch
+
+    from test_gpu_parity import EPS, TOL_EIG, TOL_PHASE
+
+    if cond:
+        A, B = oracle.gen_generic_cond(n, 48, seed=5, cond=cond)
+    else:
+        A, B = oracle.gen_generic(n, 48, seed=5)
+    s = geneig.GeneigMixedSolver(n)
+    W, U, info = geneig.empty_outputs(n, 48, device)
+    iters = torch.zeros(48, dtype=torch.int32, device=device)
+    fb = s.launch_mixed(to_dev(A, device), to_dev(B, device), W, U, info, iters=iters)
+    torch.cuda.synchronize()
+    assert (to_host(info) == 0).all()
+    Wo, Xo, io = oracle.zhegv(A, B)
+    W2, U2, i2 = geneig.empty_outputs(n, 48, device)
+    geneig.GeneigSolver(n).launch(to_dev(A, device), to_dev(B, device), W2, U2, i2)
+    torch.cuda.synchronize()
+    mp = oracle.compare(A, B, to_host(W2), to_host(U2), Wo, Xo)
+    mm = oracle.compare(A, B, to_host(W), to_host(U), Wo, Xo)
+    print("n=%d cond %.0e: fallback %d of 48, steps %s" % (n, cond, fb, hist(to_host(iters))), flush=True)
+    bars = {"eig_rel_scaled": TOL_EIG, "phase_err": TOL_PHASE, "resid": 50 * n * EPS, "orth": 50 * n * EPS}
+    for k, bar in bars.items():
+        print("  %-15s mixed %.2e  fp64 path %.2e  bar %.1e" % (k, mm[k], mp[k], bar), flush=True)
+        assert mm[k] <= max(bar, 2.0 * mp[k]), k
+    if cond <= 1e4:
+        assert fb == 0
