@@ -23,7 +23,7 @@ int geneig_gen_generic_launch(int n, int batch, uint64_t seed, uint64_t first_pa
 /* A = eta = diag(I_nsub, -I_nsub), B = H0(q) for q on a g0 x g1 x g2 grid,
  * q index first_q + b.  H0 = [[Aq, Bq], [conj Bq, conj Aq]]: bond b joins
  * site i to site (i + shift_b) mod nsub through the integer offset r_b with
- * coupling J_{b,i} ~ U(0.5, 1.5) * jscale; Aq gets J/2 e^{+-i theta} (Hermitian)
+ * coupling J_{b,i} ~ U(0.5, 1.5) * jscale (jscale < 0: * s per q, s = 0.8 + 0.4 U); Aq gets J/2 e^{+-i theta} (Hermitian)
  * plus J on the diagonal, Bq gets J/2 e^{i theta} on (i,j) and (j,i).
  * Finish with geneig_spinwave_shift_launch once lambda_min(H0) is known. */
 int geneig_gen_spinwave_h0_launch(int nsub, int nbonds, uint64_t seed, int g0, int g1, int g2,
@@ -36,6 +36,11 @@ int geneig_spinwave_shift_launch(int n, int batch, uint64_t seed, uint64_t first
                                  void* d_B, hipStream_t stream);
 
 /* B = I for every pair (used to get lambda(H0) from the (H0, I) problem). */
+/* Per-pair parameters (q0, q1, q2, s, t) of the spin-wave family as float32
+ * rows of 5: the grid point, the coupling scale (jscale, or per q when
+ * jscale < 0) and the lambda_min(H) target. */
+int geneig_spinwave_params_launch(uint64_t seed, int g0, int g1, int g2, uint64_t first_q, int batch,
+                                  double jscale, float* d_params, hipStream_t stream);
 int geneig_fill_identity_launch(int n, int batch, void* d_B, hipStream_t stream);
 
 #ifdef __cplusplus

@@ -232,7 +232,9 @@ int go_gen_spinwave(int nsub, int nbonds, uint64_t seed, int g0, int g1, int g2,
         go_spinwave_q(g0, g1, g2, qi, q);
         go_cplx* Hb = B + nn * b;
         go_cplx* Ab = A + nn * b;
-        go_spinwave_H0(nsub, nbonds, bonds, seed, jscale, q, Hb);
+        /* jscale < 0: a coupling scale per q, s = 0.8 + 0.4 U (stream 8) */
+        const double js = jscale < 0.0 ? 0.8 + 0.4 * go_u01(seed, 8, qi, 0) : jscale;
+        go_spinwave_H0(nsub, nbonds, bonds, seed, js, q, Hb);
         double* w = (double*)malloc(sizeof(double) * (size_t)n);
         if (go_heevals(n, Hb, w) != 0) bad++;
         double shift = go_spinwave_target(seed, qi) - w[0];

@@ -66,7 +66,8 @@ void go_gen_generic_cond(int n, int batch, uint64_t seed, uint64_t first_pair,
 typedef struct { int shift; int r[3]; double J; } go_bond;
 void go_spinwave_bonds(int nsub, int nbonds, uint64_t seed, go_bond* out);
 void go_spinwave_q(int g0, int g1, int g2, uint64_t qidx, double q[3]);
-/* H0(q) = [[Aq, Bq], [conj Bq, conj Aq]] scaled by jscale (n = 2 nsub). */
+/* H0(q) = [[Aq, Bq], [conj Bq, conj Aq]] scaled by jscale (n = 2 nsub); go_gen_spinwave
+ * draws a scale per q, s = 0.8 + 0.4 U (stream 8), when jscale < 0. */
 void go_spinwave_H0(int nsub, int nbonds, const go_bond* bonds, uint64_t seed,
                     double jscale, const double q[3], go_cplx* H);
 double go_spinwave_J(uint64_t seed, int b, int i);

@@ -57,6 +57,19 @@ def lib():
         "geneig_mixed_default_max_iter": (I, []),
         "geneig_mixed_default_tol": (D, []),
         "geneig_mixed_solver_create": (I, [P, I, I, I]),
+        "geneig_spinwave_params_launch": (I, [U64, I, I, I, U64, I, D, P, P]),
+        "geneig_learn_num_features": (I, []),
+        "geneig_learn_hidden": (I, []),
+        "geneig_learn_num_outputs": (I, []),
+        "geneig_learn_features": (I, [P, P, I, P]),
+        "geneig_learn_gemm": (I, [I, I, I, I, I, P, I, P, I, P, I, P, I, P]),
+        "geneig_learn_tanh_backward": (I, [P, P, C.c_longlong, P]),
+        "geneig_learn_colsum": (I, [P, I, I, P, P]),
+        "geneig_learn_adam": (I, [P, P, P, P, C.c_longlong, C.c_float, C.c_float, C.c_float, C.c_float, I, P]),
+        "geneig_learn_mlp_forward": (I, [P, P, I, P, P, P]),
+        "geneig_learn_loss": (I, [P, P, P, I, C.c_float, P, P, P]),
+        "geneig_learn_reduce_n54": (I, [P, P, P, P, P, I, P]),
+        "geneig_learned_n54_batched_launch": (I, [P, P, P, P, P, I, P, P, I, C.c_float, I, D, P, P, P, P]),
         "geneig_mixed_solver_launch": (I, [P, P, P, P, P, P, I, I, D, P, P, P]),
         "geneig_solver_create": (I, [P, I, I, I, I, I, I]),
         "geneig_solver_destroy": (None, [P]),
@@ -424,6 +437,17 @@ def gen_generic(n, batch, seed, first_pair=0, device="cuda", stream=None):
     return A, B
 
 
+def spinwave_params(batch, seed, grid=(64, 32, 32), first_q=0, jscale=1.0, device="cuda", stream=None):
+    """Per-pair (q0, q1, q2, s, t) of the spin-wave family as a (batch, 5) float32
+    tensor: grid point, coupling scale (per q when jscale < 0), lambda_min target."""
+    torch = _torch()
+    out = torch.empty((batch, 5), dtype=torch.float32, device=device)
+    _check_hip(lib().geneig_spinwave_params_launch(int(seed), grid[0], grid[1], grid[2], int(first_q), int(batch),
+                                                   float(jscale), _ptr(out), _stream(stream)),
+               "geneig_spinwave_params_launch")
+    return out
+
+
 def gen_spinwave(nsub, batch, seed, grid=(64, 32, 32), first_q=0, nbonds=12, jscale=1.0, device="cuda",
                  chunk=8192, stream=None):
     """Spin-wave pairs (eta, H(q)) with lambda_min(H) = 10^U(-3,-1) per q.
@@ -461,3 +485,6 @@ def gen_spinwave(nsub, batch, seed, grid=(64, 32, 32), first_q=0, nbonds=12, jsc
         raise GeneigError("spin-wave generator: lambda_min solve reported info != 0")
     del Hc, Ic, Wc, Uc, infoc
     return A, B
+
+
+from . import learn  # noqa: E402  (learned warm start; needs the names above)

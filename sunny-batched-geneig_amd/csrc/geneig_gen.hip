@@ -111,6 +111,8 @@ __global__ void __launch_bounds__(256) gen_spinwave_kernel(int nsub, int nbonds,
     const double q0 = (double)((qi / ((uint64_t)g2 * g1)) % (uint64_t)g0) / g0;
     const double q1 = (double)((qi / (uint64_t)g2) % (uint64_t)g1) / g1;
     const double q2 = (double)(qi % (uint64_t)g2) / g2;
+    // jscale < 0: a coupling scale per q, s = 0.8 + 0.4 U (stream 8)
+    const double js = jscale < 0.0 ? 0.8 + 0.4 * u01(seed, 8, qi, 0) : jscale;
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) H[t] = make_double2(0.0, 0.0);
     __syncthreads();
     // Each (bond, site) term touches fixed entries; serialize per bond so no two
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(256) gen_spinwave_kernel(int nsub, int nbonds,
         sincos(th, &sn, &cs);
         if (threadIdx.x == 0) {
             for (int i = 0; i < N; ++i) {
-                const double J = jscale * (0.5 + u01(seed, 6, (uint64_t)bd, (uint32_t)i));
+                const double J = js * (0.5 + u01(seed, 6, (uint64_t)bd, (uint32_t)i));
                 const int j = (i + bo.shift) % N;
                 const double h = 0.5 * J;
                 // Aq (top-left) and conj(Aq) (bottom-right)
@@ -161,6 +163,20 @@ __global__ void shift_kernel(int n, uint64_t seed, uint64_t first_q, const doubl
     const double t = pow(10.0, -3.0 + 2.0 * u01(seed, 7, first_q + (uint64_t)b, 0));
     const double sh = t - W[(size_t)b * n];
     for (int i = threadIdx.x; i < n; i += blockDim.x) B[(size_t)b * n * n + i + (size_t)i * n].x += sh;
+}
+
+// Per-pair parameters of the spin-wave family: q (3), the coupling scale s
+// and the lambda_min target t, as float32 rows of 5.
+__global__ void spinwave_params_kernel(uint64_t seed, int g0, int g1, int g2, uint64_t first_q, double jscale,
+                                       float* __restrict__ out, int batch) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    const uint64_t qi = first_q + (uint64_t)b;
+    out[5 * b + 0] = (float)((double)((qi / ((uint64_t)g2 * g1)) % (uint64_t)g0) / g0);
+    out[5 * b + 1] = (float)((double)((qi / (uint64_t)g2) % (uint64_t)g1) / g1);
+    out[5 * b + 2] = (float)((double)(qi % (uint64_t)g2) / g2);
+    out[5 * b + 3] = (float)(jscale < 0.0 ? 0.8 + 0.4 * u01(seed, 8, qi, 0) : jscale);
+    out[5 * b + 4] = (float)pow(10.0, -3.0 + 2.0 * u01(seed, 7, qi, 0));
 }
 
 __global__ void identity_kernel(int n, double2* __restrict__ B, long long total) {
@@ -210,6 +226,15 @@ int geneig_spinwave_shift_launch(int n, int batch, uint64_t seed, uint64_t first
     hipLaunchKernelGGL(shift_kernel, dim3((batch + 3) / 4), dim3(64, 4), 0, stream, n, seed, first_q, d_W,
                        static_cast<double2*>(d_B), batch);
     return (int)hipGetLastError();
+}
+
+int geneig_spinwave_params_launch(uint64_t seed, int g0, int g1, int g2, uint64_t first_q, int batch,
+                                  double jscale, float* d_params, hipStream_t stream) {
+    if (batch <= 0) return hipSuccess;
+    if (g0 <= 0 || g1 <= 0 || g2 <= 0 || !d_params) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(spinwave_params_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, stream, seed, g0,
+                       g1, g2, first_q, jscale, d_params, batch);
+    return hipGetLastError();
 }
 
 int geneig_fill_identity_launch(int n, int batch, void* d_B, hipStream_t stream) {

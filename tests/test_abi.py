@@ -11,7 +11,7 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("geneig.h", "geneig_solver.h", "geneig_gen.h", "geneig_mixed.h", "geneig_multi.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("geneig.h", "geneig_solver.h", "geneig_gen.h", "geneig_mixed.h", "geneig_multi.h", "geneig_learn.h")]
 
 
 def declared_functions():
