@@ -198,6 +198,88 @@ def bench_mixed_sizes(args, geneig, torch, dev):
     print(json.dumps(line), flush=True)
 
 
+def bench_learned(args, geneig, torch, dev):
+    """Learned warm start on the n = 54 spin-wave family with a coupling scale
+    per q (s ~ U(0.8, 1.2)): train on 32,768 pairs (32^3 q grid), evaluate on
+    262,144 unseen q (64^3 grid, same lattice), against the same pipeline from
+    Q = I, the precision form and the plain FP64 form on the same pairs."""
+    L = geneig.learn
+    fam = 11
+    t0 = time.perf_counter()
+    A, B, P = L.spinwave_family(32768, fam, (32, 32, 32))
+    Cm, info = L.reduced_matrices(A, B)
+    X = L.features(P)
+    del A, B
+    torch.cuda.synchronize()
+    log("training set ready in %.1f s (info nonzero %d)" % (time.perf_counter() - t0, int((info != 0).sum().item())))
+    m = L.WarmStart(seed=args.seed, device=dev)
+    t0 = time.perf_counter()
+    hist = m.train(X, Cm, epochs=args.train_epochs, batch=256, lr=2e-3, lr_final=1e-4, seed=args.seed,
+                   log=lambda msg: log(msg))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del Cm, X
+    nev = args.eval_batch
+    Ae, Be, Pe = L.spinwave_family(nev, fam, (64, 64, 64))
+    Xe = L.features(Pe)
+    W, U, info = geneig.empty_outputs(54, nev, dev)
+    sw = torch.zeros(nev, dtype=torch.int32, device=dev)
+    it = torch.zeros(nev, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    sel = torch.from_numpy(np.linspace(0, nev - 1, num=min(args.check, nev)).astype(np.int64)).to(dev)
+    Ah, Bh = Ae.index_select(0, sel).cpu().numpy(), Be.index_select(0, sel).cpu().numpy()
+    out = {}
+
+    def run(name, step, steps):
+        step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            r = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        acc = accuracy(Ah, Bh, W.index_select(0, sel).cpu().numpy(), U.index_select(0, sel).cpu().numpy())
+        out[name] = {"solves_per_s": nev / dt, "ms_per_step": dt * 1e3, "fallback": r,
+                     "max_eig_err_over_max_abs_eig": acc["max_eig_err_over_max_abs_eig"],
+                     "max_resid_over_n_eps": acc["max_resid_over_n_eps"],
+                     "max_orth_over_n_eps": acc["max_orth_over_n_eps"]}
+        log("%s: %s" % (name, out[name]))
+
+    steps = max(1, min(args.steps, 3))
+    run("learned", lambda: L.learned_n54_batched(Ae, Be, W, U, info, X=Xe, model=m, sweeps=sw, iters=it,
+                                                  max_sweeps=30), steps)
+    sw_l = sw.cpu().numpy()
+    it_l = it.cpu().numpy()
+    run("jacobi_from_identity", lambda: L.learned_n54_batched(Ae, Be, W, U, info, sweeps=sw, iters=it, max_sweeps=30),
+        steps)
+    sw_i = sw.cpu().numpy()
+    run("precision", lambda: geneig.mixed_n54_batched(Ae, Be, W, U, info), steps)
+    run("plain", lambda: (geneig.full_n54_batched(Ae, Be, W, U, info), 0)[1], steps)
+
+    def hist_of(a):
+        v, c = np.unique(a, return_counts=True)
+        return {str(int(k)): int(n) for k, n in zip(v, c)}
+
+    line = {
+        "metric": "learned warm start: n=54 spin-wave solves/s on unseen q, Jacobi sweeps saved, fallback rate",
+        "value": out["learned"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": out["learned"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 network + c64 Jacobi + f64 refinement",
+        "data": "synthetic (device-generated spin-wave family, lattice seed %d, s ~ U(0.8,1.2) per q)" % fam,
+        "config": {"workload": "train 32,768 pairs (32^3 q grid), evaluate %d unseen q (64^3 grid)" % nev,
+                   "network": "8 -> 256 -> 256 -> 256 (tanh) -> 5832, FP32", "train_epochs": args.train_epochs,
+                   "train_batch": 256, "train_seconds": train_s, "parallelism": "none (1 GPU)"},
+        "train_loss_first_last": [hist[0], hist[-1]],
+        "jacobi_sweeps_mean": {"network": float(sw_l.mean()), "identity": float(sw_i.mean())},
+        "jacobi_sweeps_saved_mean": float(sw_i.mean() - sw_l.mean()),
+        "jacobi_sweeps_histogram": {"network": hist_of(sw_l), "identity": hist_of(sw_i)},
+        "refine_steps_histogram_network": hist_of(it_l),
+        "fallback_rate": out["learned"]["fallback"] / nev,
+        "forms": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,9 +290,12 @@ def main():
     ap.add_argument("--check", type=int, default=256, help="pairs checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["spinwave54", "mixed-sizes"], default="spinwave54",
+    ap.add_argument("--workload", choices=["spinwave54", "mixed-sizes", "learned"], default="spinwave54",
                     help="spinwave54: the headline metric; mixed-sizes: 200,000 generic pairs over "
-                         "n in {8,...,96}, one graph launch vs one launch per size")
+                         "n in {8,...,96}, one graph launch vs one launch per size; learned: train the "
+                         "warm-start network on 32,768 solver-produced C, evaluate on 262,144 unseen q")
+    ap.add_argument("--train-epochs", type=int, default=40)
+    ap.add_argument("--eval-batch", type=int, default=262144)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -238,6 +323,9 @@ def main():
     geneig = load_package()
     if args.workload == "mixed-sizes":
         bench_mixed_sizes(args, geneig, torch, dev)
+        return
+    if args.workload == "learned":
+        bench_learned(args, geneig, torch, dev)
         return
     n, nsub, batch = 54, 27, args.batch
     grid = (64, 64, 32)
