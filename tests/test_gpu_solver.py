@@ -134,17 +134,27 @@ def test_bpb_that_does_not_fit_raises(geneig, device, cache_dir):
         geneig.GeneigSolver(54, batches_per_block_request=4)
 
 
-def test_runtime_matches_static_n54_bitwise(geneig, device, cache_dir):
-    """The hipRTC module and the hipcc build of the same source agree bit for bit."""
+def test_runtime_matches_static_n54(geneig, oracle, device, cache_dir):
+    """The hipRTC module (compiled for the device's full target) and the hipcc
+    build of the same source agree on the eigenvalues to rounding and both
+    meet the FP64 bar against the oracle."""
     import torch
+
+    from test_gpu_parity import check_pairs
 
     A, B = geneig.gen_spinwave(27, 64, seed=4)
     W1, U1, i1 = geneig.empty_outputs(54, 64, device)
     geneig.full_n54_batched(A, B, W1, U1, i1)
     W2, U2, i2 = run(geneig, geneig.GeneigSolver(54, batches_per_block_request=1), A, B, device)
     torch.cuda.synchronize()
-    assert W1.cpu().numpy().tobytes() == W2.tobytes()
-    assert U1.cpu().numpy().tobytes() == U2.tobytes()
+    W1h, U1h = W1.cpu().numpy(), U1.cpu().numpy()
+    dw = np.abs(W1h - W2).max() / np.abs(W1h).max()
+    print("static vs hipRTC FP64: max rel diff W %.2e" % dw, flush=True)
+    assert dw <= 1e-13
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    check_pairs(oracle, Ah, Bh, W1h, U1h, Wo, Xo, 54, "static")
+    check_pairs(oracle, Ah, Bh, W2, U2, Wo, Xo, 54, "hipRTC")
 
 
 def test_launch_chunked_streams(geneig, oracle, device, cache_dir):
