@@ -35,9 +35,24 @@ __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __res
     geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, smem, vecs);
 }
 
-__global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+// K2 in three kernels: the QL chases of 64 pairs per wave with the rotations
+// logged (K2a), their replay into Z (K2b), and the one-pair-per-workgroup QL
+// for the rare pair whose log overflowed (K2x).
+constexpr unsigned kK2aLds = geneig::k2a_lds_bytes<kN, double>();
+
+__global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    geneig::k2a_batched<kN, double>(info, ws, batch, reinterpret_cast<double*>(smem));
+}
+
+__global__ void __launch_bounds__(64 * C54::R, GENEIG_K2B_WAVES_PER_EU) k2b_kernel(const int* __restrict__ info,
+                                                                                 unsigned char* ws, int batch) {
+    geneig::k2b_batched<kN, double>(info, ws, batch);
+}
+
+__global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2x_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
     __shared__ geneig::cplx rotlog[C54::R * C54::K2_VEC];
-    geneig::k2_batched<kN>(info, ws, batch, rotlog);
+    geneig::k2x_batched<kN>(info, ws, batch, rotlog);
 }
 
 template <int MODE>
@@ -78,6 +93,7 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), C54::BYTES * kBPB));
 
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), C54::BYTES * kBPB));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
     if (batch <= 0) return hipSuccess;
     constexpr bool eig = (MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG ||
                           MODE == geneig::MODE_FINISH);
@@ -96,8 +112,11 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
                            b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
         if (!eig) continue;
-        hipLaunchKernelGGL(k2_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream,
-                           info + off, w, P);
+        hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, stream, info + off, w, P);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream, info + off, w, P);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream, info + off, w, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), C54::BYTES * kBPB, stream, b + mat * off,
                            W + (size_t)kN * off, u + mat * off, info + off, w, P);

@@ -4,7 +4,8 @@
 //   C0   FP64 Cholesky of B, L stashed in U (zhegv numbering and NaN outputs
 //        when B is not positive definite)
 //   K1f  complex64 two-sided reduction, Householder tridiagonalization, Q
-//   K2f  float implicit QL
+//   K2f  float implicit QL (K2a chases with a rotation log, K2b replay, K2x
+//        for an overflowed log)
 //   K3f  sort, Y = Q Z, X0 = L^{-H} Y (complex64 arithmetic, complex128 out)
 //   RF   FP64 refinement on the matrix cores; pairs it cannot finish go to a
 //        device list
@@ -58,10 +59,25 @@ __global__ void __launch_bounds__(64) k1f_kernel(const cplx* __restrict__ A, con
     geneig::k1_batched<kN, 1, geneig::MODE_MIXED, float>(A, B, W, U, info, ws, batch, smem, vecs);
 }
 
-__global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2f_kernel(int* __restrict__ info,
-                                                                                unsigned char* ws, int batch) {
+// K2 as the FP64 path runs it: chases of 64 pairs per wave with the rotations
+// logged (K2a), replay into Z (K2b), one-pair QL for an overflowed log (K2x).
+constexpr unsigned kK2aLdsF = geneig::k2a_lds_bytes<kN, float>();
+constexpr unsigned kK2aLdsD = geneig::k2a_lds_bytes<kN, double>();
+
+__global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    geneig::k2a_batched<kN, float>(info, ws, batch, reinterpret_cast<float*>(smem));
+}
+
+__global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2bf_kernel(const int* __restrict__ info,
+                                                                                 unsigned char* ws, int batch) {
+    geneig::k2b_batched<kN, float>(info, ws, batch);
+}
+
+__global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2xf_kernel(int* __restrict__ info,
+                                                                                 unsigned char* ws, int batch) {
     __shared__ cplxf rotlog[CF::R * CF::K2_VEC];
-    geneig::k2_batched<kN, float>(info, ws, batch, rotlog);
+    geneig::k2x_batched<kN, float>(info, ws, batch, rotlog);
 }
 
 __global__ void __launch_bounds__(64) k3f_kernel(const cplx* __restrict__ B, double* __restrict__ W,
@@ -91,11 +107,23 @@ __global__ void __launch_bounds__(64) k1l_kernel(const cplx* __restrict__ A, con
     geneig::k1_batched<kN, 1, geneig::MODE_FULL, double>(A, B, W, U, info, ws, count, smem, vecs, list);
 }
 
-__global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2l_kernel(int* __restrict__ info,
-                                                                               unsigned char* ws, int count,
-                                                                               const int* __restrict__ list) {
+__global__ void __launch_bounds__(64) k2al_kernel(int* __restrict__ info, unsigned char* ws, int count,
+                                                   const int* __restrict__ list) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    geneig::k2a_batched<kN, double>(info, ws, count, reinterpret_cast<double*>(smem), list);
+}
+
+__global__ void __launch_bounds__(64 * CD::R, GENEIG_K2B_WAVES_PER_EU) k2bl_kernel(const int* __restrict__ info,
+                                                                                 unsigned char* ws, int count,
+                                                                                 const int* __restrict__ list) {
+    geneig::k2b_batched<kN, double>(info, ws, count, list);
+}
+
+__global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2xl_kernel(int* __restrict__ info,
+                                                                                unsigned char* ws, int count,
+                                                                                const int* __restrict__ list) {
     __shared__ cplx rotlog[CD::R * CD::K2_VEC];
-    geneig::k2_batched<kN, double>(info, ws, count, rotlog, list);
+    geneig::k2x_batched<kN, double>(info, ws, count, rotlog, list);
 }
 
 __global__ void __launch_bounds__(64) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
@@ -137,6 +165,8 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), CD::BYTES));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2af_kernel), kK2aLdsF));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2al_kernel), kK2aLdsD));
     if (batch == 0) return hipSuccess;
     if (maxit <= 0) maxit = kMaxIter;
     if (!(tol > 0.0)) tol = kTol;
@@ -162,7 +192,12 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, A + mat * off, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, solver, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2f_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
+        hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, stream, info + off,
+                           solver, P);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k2bf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, solver, P);
@@ -182,7 +217,13 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), CD::BYTES, stream, A, B, W, U, info, solver, P,
                            fb_list + off);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2l_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
+        hipLaunchKernelGGL(k2al_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsD, stream, info, solver,
+                           P, fb_list + off);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k2bl_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
+                           fb_list + off);
+        RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k2xl_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
                            fb_list + off);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), CD::BYTES, stream, B, W, U, info, solver, P,

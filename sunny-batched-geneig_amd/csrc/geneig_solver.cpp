@@ -191,16 +191,20 @@ int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vecto
 struct Module {
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;    // K1: factor, reduce, tridiagonalize, form Q
-    hipFunction_t fn2 = nullptr;   // K2: QL with the rotation product in registers
+    hipFunction_t fn2a = nullptr;  // K2a: QL chases, 64 pairs per wave, rotations logged
+    hipFunction_t fn2b = nullptr;  // K2b: Z replayed from the log
+    hipFunction_t fn2x = nullptr;  // K2x: one-pair QL for a pair whose log overflowed
     hipFunction_t fn3 = nullptr;   // K3: sort, Y = Q Z, back-transform
     unsigned lds_per_pair = 0;     // K1 / K3 dynamic LDS bytes per pair
     unsigned lds_total = 0;        // K1 / K3 dynamic + static LDS bytes per pair
     unsigned k2_threads = 0;
+    unsigned k2a_lds = 0, f_k2a_lds = 0;   // K2a dynamic LDS (FP64, complex64 form)
     unsigned long long ws_pair_bytes = 0;
     // precision form (variant 1): FP64 Cholesky, complex64 K1/K2/K3, FP64
     // refinement, FP64 K1/K2/K3 over a pair list, counter clear
-    hipFunction_t fc = nullptr, f1f = nullptr, f2f = nullptr, f3f = nullptr, fr = nullptr;
-    hipFunction_t f1l = nullptr, f2l = nullptr, f3l = nullptr, fz = nullptr;
+    hipFunction_t fc = nullptr, f1f = nullptr, f2af = nullptr, f2bf = nullptr, f2xf = nullptr, f3f = nullptr;
+    hipFunction_t fr = nullptr, f1l = nullptr, f2al = nullptr, f2bl = nullptr, f2xl = nullptr, f3l = nullptr;
+    hipFunction_t fz = nullptr;
     unsigned f_lds_per_pair = 0, f_k2_threads = 0, refine_lds = 0, refine_threads = 0;
     unsigned long long f_ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
@@ -222,16 +226,22 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     if (variant == 1) {
         HIP_TRY(hipModuleGetFunction(&m->fc, m->mod, "geneig_chol64"));
         HIP_TRY(hipModuleGetFunction(&m->f1f, m->mod, "geneig_k1f"));
-        HIP_TRY(hipModuleGetFunction(&m->f2f, m->mod, "geneig_k2f"));
+        HIP_TRY(hipModuleGetFunction(&m->f2af, m->mod, "geneig_k2af"));
+        HIP_TRY(hipModuleGetFunction(&m->f2bf, m->mod, "geneig_k2bf"));
+        HIP_TRY(hipModuleGetFunction(&m->f2xf, m->mod, "geneig_k2xf"));
         HIP_TRY(hipModuleGetFunction(&m->f3f, m->mod, "geneig_k3f"));
         HIP_TRY(hipModuleGetFunction(&m->fr, m->mod, "geneig_refine"));
         HIP_TRY(hipModuleGetFunction(&m->f1l, m->mod, "geneig_k1l"));
-        HIP_TRY(hipModuleGetFunction(&m->f2l, m->mod, "geneig_k2l"));
+        HIP_TRY(hipModuleGetFunction(&m->f2al, m->mod, "geneig_k2al"));
+        HIP_TRY(hipModuleGetFunction(&m->f2bl, m->mod, "geneig_k2bl"));
+        HIP_TRY(hipModuleGetFunction(&m->f2xl, m->mod, "geneig_k2xl"));
         HIP_TRY(hipModuleGetFunction(&m->f3l, m->mod, "geneig_k3l"));
         HIP_TRY(hipModuleGetFunction(&m->fz, m->mod, "geneig_zero"));
     } else {
         HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_k1"));
-        HIP_TRY(hipModuleGetFunction(&m->fn2, m->mod, "geneig_k2"));
+        HIP_TRY(hipModuleGetFunction(&m->fn2a, m->mod, "geneig_k2a"));
+        HIP_TRY(hipModuleGetFunction(&m->fn2b, m->mod, "geneig_k2b"));
+        HIP_TRY(hipModuleGetFunction(&m->fn2x, m->mod, "geneig_k2x"));
         HIP_TRY(hipModuleGetFunction(&m->fn3, m->mod, "geneig_k3"));
     }
     auto read_global = [&](const char* name, void* dst, size_t want) -> int {
@@ -245,10 +255,12 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     int rc = read_global("geneig_lds_bytes_per_pair", &m->lds_per_pair, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_lds_total_per_pair", &m->lds_total, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_k2_threads", &m->k2_threads, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_k2a_lds_bytes", &m->k2a_lds, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
     if (variant == 1) {
         if (!rc) rc = read_global("geneig_f_lds_bytes_per_pair", &m->f_lds_per_pair, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2_threads", &m->f_k2_threads, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_f_k2a_lds_bytes", &m->f_k2a_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_ws_bytes_per_pair", &m->f_ws_pair_bytes, sizeof(unsigned long long));
         if (!rc) rc = read_global("geneig_refine_lds_bytes", &m->refine_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_refine_threads", &m->refine_threads, sizeof(unsigned));
@@ -428,7 +440,10 @@ struct geneig_solver {
             const unsigned g13 = (unsigned)((P + b - 1) / b);
             HIP_TRY(hipModuleLaunchKernel(m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1, nullptr));
             void* a2[] = {&inf, &ws, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.fn2, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.fn2a, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2,
+                                          nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.fn2b, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
             HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a3, nullptr));
         }
@@ -469,7 +484,10 @@ struct geneig_solver {
             void* a1[] = {&h, &sp, &w, &u, &inf, &solver, &P};
             HIP_TRY(hipModuleLaunchKernel(m.f1f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a1, nullptr));
             void* a2[] = {&inf, &solver, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.f2f, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f2af, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.f_k2a_lds, stream, a2,
+                                          nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f2bf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f2xf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &solver, &P};
             HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a3, nullptr));
         }
@@ -487,7 +505,10 @@ struct geneig_solver {
             void* a1[] = {&H, &S, &W, &U, &info, &solver, &P, &lst};
             HIP_TRY(hipModuleLaunchKernel(m.f1l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a1, nullptr));
             void* a2[] = {&info, &solver, &P, &lst};
-            HIP_TRY(hipModuleLaunchKernel(m.f2l, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f2al, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2,
+                                          nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f2bl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f2xl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&S, &W, &U, &info, &solver, &P, &lst};
             HIP_TRY(hipModuleLaunchKernel(m.f3l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a3, nullptr));
         }
@@ -506,9 +527,13 @@ struct geneig_solver {
             set(m.f1f, m.f_lds_per_pair);
             set(m.f3f, m.f_lds_per_pair);
             set(m.fr, m.refine_lds);
+            set(m.f2af, m.f_k2a_lds);
+            set(m.f2al, m.k2a_lds);
             return;
         }
         const int bytes = (int)(m.lds_per_pair * (unsigned)b);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn2a), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)m.k2a_lds);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn3), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     }
