@@ -22,7 +22,10 @@ std::map<std::pair<int, hipStream_t>, Buf> g_bufs;
 size_t budget_bytes() {
     static const size_t b = [] {
         const char* e = std::getenv("GENEIG_WORKSPACE_MB");
-        long mb = e ? std::atol(e) : 1024;
+        // 8 GiB by default: the QL stage runs 64 pairs per wave as serial
+        // chains, so it needs tens of thousands of pairs per dispatch to
+        // occupy the chip, and 8 GiB is under 3% of an MI355X's HBM.
+        long mb = e ? std::atol(e) : 8192;
         if (mb < 16) mb = 16;
         return (size_t)mb << 20;
     }();

@@ -191,7 +191,7 @@ m = O.compare(Ah, Bh, W.cpu().numpy(), U.cpu().numpy(), Wo, Xo)
 assert int(info.abs().max()) == 0 and m["eig_rel_scaled"] < 1e-12 and m["phase_err"] < 1e-10, m
 print("ok", m["eig_rel_scaled"])
 ''' % (os.path.join(os.path.dirname(GOLDEN)), os.path.join(os.path.dirname(os.path.dirname(GOLDEN)), "oracle"))
-    env = dict(os.environ, GENEIG_WORKSPACE_MB="16")   # 16 MiB -> chunks of ~236 pairs
+    env = dict(os.environ, GENEIG_WORKSPACE_MB="16")   # 16 MiB -> chunks of 141 pairs at n = 54
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
     print(out.stdout, out.stderr[-2000:], flush=True)
     assert out.returncode == 0 and "ok" in out.stdout
