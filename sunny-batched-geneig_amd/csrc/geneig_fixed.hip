@@ -39,6 +39,7 @@ __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __res
 // logged (K2a), their replay into Z (K2b), and the one-pair-per-workgroup QL
 // for the rare pair whose log overflowed (K2x).
 constexpr unsigned kK2aLds = geneig::k2a_lds_bytes<kN, double>();
+constexpr unsigned kK3Lds = geneig::k3_lds_bytes<kN, double>();
 
 __global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -56,7 +57,7 @@ __global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2x_kerne
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(64 * kBPB) k3_kernel(const geneig::cplx* __restrict__ B,
+__global__ void __launch_bounds__(64 * kBPB, GENEIG_K3_WAVES_PER_EU) k3_kernel(const geneig::cplx* __restrict__ B,
                                                         double* __restrict__ W, geneig::cplx* __restrict__ U,
                                                         const int* __restrict__ info, unsigned char* ws,
                                                         int batch) {
@@ -92,7 +93,7 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
     auto k3 = k3_kernel<MODE>;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), C54::BYTES * kBPB));
 
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), C54::BYTES * kBPB));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), kK3Lds * kBPB));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
     if (batch <= 0) return hipSuccess;
     constexpr bool eig = (MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG ||
@@ -118,7 +119,7 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), C54::BYTES * kBPB, stream, b + mat * off,
+        hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), kK3Lds * kBPB, stream, b + mat * off,
                            W + (size_t)kN * off, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
     }

@@ -44,6 +44,8 @@ using CF = geneig::Cfg<kN, float>;
 using WD = geneig::Ws<kN, double>;
 using WF = geneig::Ws<kN, float>;
 using RC = geneig::RCfg<kN>;
+constexpr unsigned kK3LdsF = geneig::k3_lds_bytes<kN, float>();
+constexpr unsigned kK3LdsD = geneig::k3_lds_bytes<kN, double>();
 
 // ---- FP32 GEMM: C = op(A) op(B) (+ bias per column) (tanh) ---------------------
 // Row-major operands.  op(A) is M x K (A stored M x K, or K x M when TA),
@@ -468,7 +470,7 @@ __global__ void __launch_bounds__(64) k1c_kernel(const cplx* __restrict__ A, con
     __shared__ cplxf vecs[CF::VEC];
     geneig::k1_batched<kN, 1, geneig::MODE_MIXED_C, float>(A, B, W, U, info, ws, batch, smem, vecs);
 }
-__global__ void __launch_bounds__(64) k3j_kernel(const cplx* __restrict__ B, double* __restrict__ W,
+__global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3j_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
                                                   unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -497,7 +499,7 @@ __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2l_kernel
     __shared__ cplx rotlog[CD::R * CD::K2_VEC];
     geneig::k2_batched<kN, double>(info, ws, count, rotlog, list);
 }
-__global__ void __launch_bounds__(64) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
+__global__ void __launch_bounds__(64, GENEIG_K3_WAVES_PER_EU) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
                                                   unsigned char* ws, int count, const int* __restrict__ list) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -606,7 +608,7 @@ int geneig_learn_mlp_forward(const float* const* d_weights, const float* d_X, in
 int geneig_learn_reduce_n54(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, hipDoubleComplex* d_L,
                             int* d_info, void* d_C, int batch, hipStream_t stream) {
     if (batch <= 0) return hipSuccess;
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), CF::BYTES));
     const int chunk = geneig_host::chunk_pairs(batch, WF::PAIR_BYTES);
     void* ws = nullptr;
@@ -614,7 +616,7 @@ int geneig_learn_reduce_n54(const hipDoubleComplex* d_H, const hipDoubleComplex*
     const auto* A = reinterpret_cast<const cplx*>(d_H);
     const auto* B = reinterpret_cast<const cplx*>(d_S);
     auto* L = reinterpret_cast<cplx*>(d_L);
-    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), CD::BYTES, stream, B, nullptr, L, d_info,
+    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B, nullptr, L, d_info,
                        batch);
     RETURN_IF(hipGetLastError());
     const size_t mat = (size_t)kN * kN;
@@ -639,13 +641,13 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
     if (h_fallback) *h_fallback = 0;
     if (batch < 0) return hipErrorInvalidValue;
     if (d_features && !d_weights) return hipErrorInvalidValue;
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), CF::BYTES));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(jacobi_kernel), kLossLds));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3j_kernel), CF::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3j_kernel), kK3LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), CD::BYTES));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     if (batch == 0) return hipSuccess;
     if (max_iter <= 0) max_iter = 4;
     if (!(tol > 0.0)) tol = 1e-7;
@@ -670,7 +672,7 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
 
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, stream, fb_count);
     RETURN_IF(hipGetLastError());
-    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), CD::BYTES, stream, B, d_W, U, d_info, batch);
+    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B, d_W, U, d_info, batch);
     RETURN_IF(hipGetLastError());
     for (int off = 0; off < batch; off += chunk_f) {
         const int P = std::min(chunk_f, batch - off);
@@ -684,7 +686,7 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
                            d_features ? (const float*)Z : nullptr, solver, P, (const int*)(d_info + off), max_sweeps,
                            jacobi_tol, d_sweeps ? d_sweeps + off : nullptr);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3j_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, B + mat * off,
+        hipLaunchKernelGGL(k3j_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, stream, B + mat * off,
                            d_W + (size_t)kN * off, U + mat * off, (const int*)(d_info + off), solver, P);
         RETURN_IF(hipGetLastError());
     }
@@ -704,7 +706,7 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
         hipLaunchKernelGGL(k2l_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, d_info, solver, P,
                            (const int*)(fb_list + off));
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), CD::BYTES, stream, B, d_W, U, (const int*)d_info,
+        hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), kK3LdsD, stream, B, d_W, U, (const int*)d_info,
                            solver, P, (const int*)(fb_list + off));
         RETURN_IF(hipGetLastError());
     }

@@ -63,6 +63,8 @@ __global__ void __launch_bounds__(64) k1f_kernel(const cplx* __restrict__ A, con
 // logged (K2a), replay into Z (K2b), one-pair QL for an overflowed log (K2x).
 constexpr unsigned kK2aLdsF = geneig::k2a_lds_bytes<kN, float>();
 constexpr unsigned kK2aLdsD = geneig::k2a_lds_bytes<kN, double>();
+constexpr unsigned kK3LdsF = geneig::k3_lds_bytes<kN, float>();
+constexpr unsigned kK3LdsD = geneig::k3_lds_bytes<kN, double>();
 
 __global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -80,7 +82,7 @@ __global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2xf_kern
     geneig::k2x_batched<kN, float>(info, ws, batch, rotlog);
 }
 
-__global__ void __launch_bounds__(64) k3f_kernel(const cplx* __restrict__ B, double* __restrict__ W,
+__global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3f_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
                                                   unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2xl_kerne
     geneig::k2x_batched<kN, double>(info, ws, count, rotlog, list);
 }
 
-__global__ void __launch_bounds__(64) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
+__global__ void __launch_bounds__(64, GENEIG_K3_WAVES_PER_EU) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
                                                   unsigned char* ws, int count, const int* __restrict__ list) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -159,12 +161,12 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
                      double tol, int* iters, int* h_fallback, hipStream_t stream) {
     if (h_fallback) *h_fallback = 0;
     if (batch < 0) return hipErrorInvalidValue;
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel), CF::BYTES));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), CF::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), kK3LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), CD::BYTES));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2af_kernel), kK2aLdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2al_kernel), kK2aLdsD));
     if (batch == 0) return hipSuccess;
@@ -185,7 +187,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
 
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, stream, fb_count);
     RETURN_IF(hipGetLastError());
-    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), CD::BYTES, stream, B, W, U, info, batch);
+    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B, W, U, info, batch);
     RETURN_IF(hipGetLastError());
     for (int off = 0; off < batch; off += chunk_f) {
         const int P = std::min(chunk_f, batch - off);
@@ -199,7 +201,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, B + mat * off,
+        hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, stream, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, solver, P);
         RETURN_IF(hipGetLastError());
     }
@@ -226,7 +228,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         hipLaunchKernelGGL(k2xl_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
                            fb_list + off);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), CD::BYTES, stream, B, W, U, info, solver, P,
+        hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), kK3LdsD, stream, B, W, U, info, solver, P,
                            fb_list + off);
         RETURN_IF(hipGetLastError());
     }

@@ -199,6 +199,8 @@ struct Module {
     unsigned lds_total = 0;        // K1 / K3 dynamic + static LDS bytes per pair
     unsigned k2_threads = 0;
     unsigned k2a_lds = 0, f_k2a_lds = 0;   // K2a dynamic LDS (FP64, complex64 form)
+    unsigned chol64_lds = 0;               // FP64 Cholesky stage dynamic LDS (precision form)
+    unsigned k3_lds = 0, f_k3_lds = 0;     // K3 dynamic LDS per pair (FP64, complex64 form)
     unsigned long long ws_pair_bytes = 0;
     // precision form (variant 1): FP64 Cholesky, complex64 K1/K2/K3, FP64
     // refinement, FP64 K1/K2/K3 over a pair list, counter clear
@@ -256,11 +258,14 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     if (!rc) rc = read_global("geneig_lds_total_per_pair", &m->lds_total, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_k2_threads", &m->k2_threads, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_k2a_lds_bytes", &m->k2a_lds, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_k3_lds_bytes", &m->k3_lds, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
     if (variant == 1) {
         if (!rc) rc = read_global("geneig_f_lds_bytes_per_pair", &m->f_lds_per_pair, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2_threads", &m->f_k2_threads, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2a_lds_bytes", &m->f_k2a_lds, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_chol64_lds_bytes", &m->chol64_lds, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_f_k3_lds_bytes", &m->f_k3_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_ws_bytes_per_pair", &m->f_ws_pair_bytes, sizeof(unsigned long long));
         if (!rc) rc = read_global("geneig_refine_lds_bytes", &m->refine_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_refine_threads", &m->refine_threads, sizeof(unsigned));
@@ -445,7 +450,8 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.fn2b, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a3, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, m.k3_lds * (unsigned)b, stream, a3,
+                                          nullptr));
         }
         return GENEIG_E_OK;
     }
@@ -473,7 +479,7 @@ struct geneig_solver {
         void* az[] = {&fb_count};
         HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 1, 1, 1, 0, stream, az, nullptr));
         void* ac[] = {&S, &W, &U, &info, &batch};
-        HIP_TRY(hipModuleLaunchKernel(m.fc, (unsigned)batch, 1, 1, 64, 1, 1, m.lds_per_pair, stream, ac, nullptr));
+        HIP_TRY(hipModuleLaunchKernel(m.fc, (unsigned)batch, 1, 1, 64, 1, 1, m.chol64_lds, stream, ac, nullptr));
         for (int off = 0; off < batch; off += chunk_f) {
             int P = std::min(chunk_f, batch - off);
             const void* h = static_cast<const char*>(H) + mat * off;
@@ -489,7 +495,7 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.f2bf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.f2xf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &solver, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a3, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, 64, 1, 1, m.f_k3_lds, stream, a3, nullptr));
         }
         void* ar[] = {&H, &S, &W, &U, &info, &fb_list, &fb_count, &iters, &batch, &maxit, &tol};
         HIP_TRY(hipModuleLaunchKernel(m.fr, (unsigned)batch, 1, 1, m.refine_threads, 1, 1, m.refine_lds, stream, ar,
@@ -510,7 +516,7 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.f2bl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.f2xl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             void* a3[] = {&S, &W, &U, &info, &solver, &P, &lst};
-            HIP_TRY(hipModuleLaunchKernel(m.f3l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a3, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f3l, (unsigned)P, 1, 1, 64, 1, 1, m.k3_lds, stream, a3, nullptr));
         }
         return GENEIG_E_OK;
     }
@@ -521,11 +527,11 @@ struct geneig_solver {
                 (void)hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)bytes);
             };
-            set(m.fc, m.lds_per_pair);
+            set(m.fc, m.chol64_lds);
             set(m.f1l, m.lds_per_pair);
-            set(m.f3l, m.lds_per_pair);
+            set(m.f3l, m.k3_lds);
             set(m.f1f, m.f_lds_per_pair);
-            set(m.f3f, m.f_lds_per_pair);
+            set(m.f3f, m.f_k3_lds);
             set(m.fr, m.refine_lds);
             set(m.f2af, m.f_k2a_lds);
             set(m.f2al, m.k2a_lds);
@@ -535,7 +541,8 @@ struct geneig_solver {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn2a), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)m.k2a_lds);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn3), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn3), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(m.k3_lds * (unsigned)b));
     }
 
     int autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands, TuneWinner& win);
