@@ -40,6 +40,10 @@ __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __res
 // for the rare pair whose log overflowed (K2x).
 constexpr unsigned kK2aLds = geneig::k2a_lds_bytes<kN, double>();
 constexpr unsigned kK3Lds = geneig::k3_lds_bytes<kN, double>();
+constexpr unsigned kK1Lds = geneig::k1_lds_bytes<kN, double>();
+// dynamic LDS of the larger of K1 and K3 plus the static broadcast vectors
+constexpr unsigned kWorkLds = kK1Lds > kK3Lds ? kK1Lds : kK3Lds;
+constexpr unsigned kVecLds = sizeof(geneig::cplx) * C54::VEC;
 
 __global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -91,7 +95,7 @@ template <int MODE>
 hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int batch, hipStream_t stream) {
     auto k1 = k1_kernel<MODE>;
     auto k3 = k3_kernel<MODE>;
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), C54::BYTES * kBPB));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), kK1Lds * kBPB));
 
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), kK3Lds * kBPB));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
@@ -100,7 +104,8 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
                           MODE == geneig::MODE_FINISH);
     const int chunk = geneig_host::chunk_pairs(batch, W54::PAIR_BYTES);
     void* ws = nullptr;
-    if (eig) RETURN_IF(geneig_host::workspace(stream, (size_t)chunk * W54::PAIR_BYTES, &ws));
+    // every mode past the Cholesky uses the workspace (K1 stages W = L^{-1} A there)
+    if (MODE != geneig::MODE_CHOL) RETURN_IF(geneig_host::workspace(stream, (size_t)chunk * W54::PAIR_BYTES, &ws));
     const size_t mat = (size_t)kN * kN;
     const auto* a = static_cast<const geneig::cplx*>(A);
     const auto* b = static_cast<const geneig::cplx*>(B);
@@ -109,7 +114,7 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
         const int P = std::min(chunk, batch - off);
         const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
         unsigned char* w = static_cast<unsigned char*>(ws);
-        hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), C54::BYTES * kBPB, stream, a + mat * off,
+        hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, stream, a + mat * off,
                            b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
         if (!eig) continue;
@@ -135,14 +140,14 @@ int chol_n54_launch(const hipDoubleComplex* d_S, hipDoubleComplex* d_L, int* d_i
     return run<geneig::MODE_CHOL>(nullptr, d_S, nullptr, d_L, d_info, 1, stream);
 }
 unsigned int chol_n54_block_dim_x(void) { return 64; }
-unsigned int chol_n54_shared_mem_bytes(void) { return C54::TOTAL; }
+unsigned int chol_n54_shared_mem_bytes(void) { return kWorkLds + kVecLds; }
 
 int geneig_reduce_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                              hipDoubleComplex* d_M, int* d_info, hipStream_t stream) {
     return run<geneig::MODE_REDUCE>(d_H, d_S, nullptr, d_M, d_info, 1, stream);
 }
 unsigned int geneig_reduce_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_reduce_n54_shared_mem_bytes(void) { return C54::TOTAL; }
+unsigned int geneig_reduce_n54_shared_mem_bytes(void) { return kWorkLds + kVecLds; }
 
 int geneig_reduce_eig_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                                  double* d_W, hipDoubleComplex* d_V, int* d_info,
@@ -150,11 +155,11 @@ int geneig_reduce_eig_n54_launch(const hipDoubleComplex* d_H, const hipDoubleCom
     return run<geneig::MODE_REDUCE_EIG>(d_H, d_S, d_W, d_V, d_info, 1, stream);
 }
 unsigned int geneig_reduce_eig_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_reduce_eig_n54_shared_mem_bytes(void) { return C54::TOTAL; }
+unsigned int geneig_reduce_eig_n54_shared_mem_bytes(void) { return kWorkLds + kVecLds; }
 // The factor lives in the U/V buffer, so the "Cholesky region" of LDS is the
 // working matrix itself; the eigen-stage region is the two broadcast vectors.
-unsigned int geneig_reduce_eig_n54_chol_smem_bytes(void) { return C54::BYTES; }
-unsigned int geneig_reduce_eig_n54_heev_smem_bytes(void) { return C54::TOTAL - C54::BYTES; }
+unsigned int geneig_reduce_eig_n54_chol_smem_bytes(void) { return kWorkLds; }
+unsigned int geneig_reduce_eig_n54_heev_smem_bytes(void) { return kVecLds; }
 unsigned int geneig_reduce_eig_n54_heev_workspace(void) { return C54::VEC; }
 
 int geneig_full_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, double* d_W,
@@ -162,7 +167,7 @@ int geneig_full_n54_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* 
     return run<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, 1, stream);
 }
 unsigned int geneig_full_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_full_n54_shared_mem_bytes(void) { return C54::TOTAL; }
+unsigned int geneig_full_n54_shared_mem_bytes(void) { return kWorkLds + kVecLds; }
 
 int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
                                    double* d_W, hipDoubleComplex* d_U, int* d_info,
@@ -170,7 +175,7 @@ int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleC
     return run<geneig::MODE_FULL>(d_H, d_S, d_W, d_U, d_info, batch_size, stream);
 }
 unsigned int geneig_full_n54_batched_block_dim_x(void) { return 64; }
-unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return C54::TOTAL * kBPB; }
+unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return (kWorkLds + kVecLds) * kBPB; }
 
 int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex* d_M, double* d_W,
                              hipDoubleComplex* d_U, int* d_info, int batch_size,
@@ -178,6 +183,6 @@ int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex
     return run<geneig::MODE_FINISH>(d_M, d_L, d_W, d_U, d_info, batch_size, stream);
 }
 unsigned int geneig_finish_n54_block_dim_x(void) { return 64; }
-unsigned int geneig_finish_n54_shared_mem_bytes(void) { return C54::TOTAL * kBPB; }
+unsigned int geneig_finish_n54_shared_mem_bytes(void) { return (kWorkLds + kVecLds) * kBPB; }
 
 }  // extern "C"

@@ -46,6 +46,8 @@ using WF = geneig::Ws<kN, float>;
 using RC = geneig::RCfg<kN>;
 constexpr unsigned kK3LdsF = geneig::k3_lds_bytes<kN, float>();
 constexpr unsigned kK3LdsD = geneig::k3_lds_bytes<kN, double>();
+constexpr unsigned kK1LdsF = geneig::k1_lds_bytes<kN, float>();
+constexpr unsigned kK1LdsD = geneig::k1_lds_bytes<kN, double>();
 
 // ---- FP32 GEMM: C = op(A) op(B) (+ bias per column) (tanh) ---------------------
 // Row-major operands.  op(A) is M x K (A stored M x K, or K x M when TA),
@@ -609,7 +611,7 @@ int geneig_learn_reduce_n54(const hipDoubleComplex* d_H, const hipDoubleComplex*
                             int* d_info, void* d_C, int batch, hipStream_t stream) {
     if (batch <= 0) return hipSuccess;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), CF::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), kK1LdsF));
     const int chunk = geneig_host::chunk_pairs(batch, WF::PAIR_BYTES);
     void* ws = nullptr;
     RETURN_IF(geneig_host::workspace(stream, (size_t)chunk * WF::PAIR_BYTES, &ws));
@@ -622,7 +624,7 @@ int geneig_learn_reduce_n54(const hipDoubleComplex* d_H, const hipDoubleComplex*
     const size_t mat = (size_t)kN * kN;
     for (int off = 0; off < batch; off += chunk) {
         const int P = std::min(chunk, batch - off);
-        hipLaunchKernelGGL(k1c_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, A + mat * off, B + mat * off,
+        hipLaunchKernelGGL(k1c_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, stream, A + mat * off, B + mat * off,
                            nullptr, L + mat * off, d_info + off, static_cast<unsigned char*>(ws), P);
         RETURN_IF(hipGetLastError());
         // the chunk's C matrices are contiguous in the workspace's Q slot
@@ -642,11 +644,11 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
     if (batch < 0) return hipErrorInvalidValue;
     if (d_features && !d_weights) return hipErrorInvalidValue;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), CF::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), kK1LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(jacobi_kernel), kLossLds));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3j_kernel), kK3LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), kK1LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     if (batch == 0) return hipSuccess;
     if (max_iter <= 0) max_iter = 4;
@@ -678,7 +680,7 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
         const int P = std::min(chunk_f, batch - off);
         float* H = reinterpret_cast<float*>(solver + (size_t)P * WF::PAIR_BYTES);
         float* Z = H + 3ull * Hd * P;
-        hipLaunchKernelGGL(k1c_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, A + mat * off, B + mat * off,
+        hipLaunchKernelGGL(k1c_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, stream, A + mat * off, B + mat * off,
                            d_W + (size_t)kN * off, U + mat * off, d_info + off, solver, P);
         RETURN_IF(hipGetLastError());
         if (d_features) RETURN_IF(mlp_forward(d_weights, d_features + (size_t)F * off, P, H, Z, stream));
@@ -700,7 +702,7 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
     if (h_fallback) *h_fallback = count;
     for (int off = 0; off < count; off += chunk_d) {
         const int P = std::min(chunk_d, count - off);
-        hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), CD::BYTES, stream, A, B, d_W, U, d_info, solver, P,
+        hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), kK1LdsD, stream, A, B, d_W, U, d_info, solver, P,
                            (const int*)(fb_list + off));
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2l_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, d_info, solver, P,

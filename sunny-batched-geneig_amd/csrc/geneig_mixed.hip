@@ -64,6 +64,8 @@ __global__ void __launch_bounds__(64) k1f_kernel(const cplx* __restrict__ A, con
 constexpr unsigned kK2aLdsF = geneig::k2a_lds_bytes<kN, float>();
 constexpr unsigned kK2aLdsD = geneig::k2a_lds_bytes<kN, double>();
 constexpr unsigned kK3LdsF = geneig::k3_lds_bytes<kN, float>();
+constexpr unsigned kK1LdsF = geneig::k1_lds_bytes<kN, float>();
+constexpr unsigned kK1LdsD = geneig::k1_lds_bytes<kN, double>();
 constexpr unsigned kK3LdsD = geneig::k3_lds_bytes<kN, double>();
 
 __global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
@@ -162,10 +164,10 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     if (h_fallback) *h_fallback = 0;
     if (batch < 0) return hipErrorInvalidValue;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel), CF::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel), kK1LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), kK3LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), CD::BYTES));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), kK1LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2af_kernel), kK2aLdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2al_kernel), kK2aLdsD));
@@ -191,7 +193,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     RETURN_IF(hipGetLastError());
     for (int off = 0; off < batch; off += chunk_f) {
         const int P = std::min(chunk_f, batch - off);
-        hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), CF::BYTES, stream, A + mat * off, B + mat * off,
+        hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, stream, A + mat * off, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, solver, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, stream, info + off,
@@ -216,7 +218,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     if (h_fallback) *h_fallback = count;
     for (int off = 0; off < count; off += chunk_d) {
         const int P = std::min(chunk_d, count - off);
-        hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), CD::BYTES, stream, A, B, W, U, info, solver, P,
+        hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), kK1LdsD, stream, A, B, W, U, info, solver, P,
                            fb_list + off);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2al_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsD, stream, info, solver,

@@ -195,8 +195,8 @@ struct Module {
     hipFunction_t fn2b = nullptr;  // K2b: Z replayed from the log
     hipFunction_t fn2x = nullptr;  // K2x: one-pair QL for a pair whose log overflowed
     hipFunction_t fn3 = nullptr;   // K3: sort, Y = Q Z, back-transform
-    unsigned lds_per_pair = 0;     // K1 / K3 dynamic LDS bytes per pair
-    unsigned lds_total = 0;        // K1 / K3 dynamic + static LDS bytes per pair
+    unsigned lds_per_pair = 0;     // K1 dynamic LDS bytes per pair
+    unsigned lds_total = 0;        // larger of K1 / K3 dynamic LDS + static vectors, per pair
     unsigned k2_threads = 0;
     unsigned k2a_lds = 0, f_k2a_lds = 0;   // K2a dynamic LDS (FP64, complex64 form)
     unsigned chol64_lds = 0;               // FP64 Cholesky stage dynamic LDS (precision form)
@@ -344,6 +344,9 @@ struct geneig_solver {
     std::string arch;
     size_t lds_per_cu = 0, lds_per_block = 0;
     std::shared_ptr<Module> module;
+    // the bpb = 1 module that sized the LDS per pair, kept so that a second
+    // solver for the same n finds both of its modules in-process
+    std::shared_ptr<Module> probe;
     unsigned shared_bytes = 0;
     int autotuned_bdx = 0;
     bool was_autotuned = false, sweep_ran = false;
@@ -745,7 +748,7 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
     int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay);
     if (rc) return rc;
     note(lay);
-    keep.reset();
+    s->probe = std::move(keep);
     s->acquire_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     s->layer = deepest == 2 ? "compile" : deepest == 1 ? "disk" : "in-process";
     s->bpb = bpb;

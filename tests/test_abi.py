@@ -50,7 +50,7 @@ def test_static_queries(geneig):
     L = geneig.lib()
     assert L.geneig_full_n54_batched_block_dim_x() == 64
     lds = L.geneig_full_n54_batched_shared_mem_bytes()
-    assert 16 * 54 * 55 <= lds <= 160 * 1024
+    assert 16 * 54 * 55 // 2 <= lds <= 160 * 1024   # the packed lower triangle of C at least
     assert L.geneig_reduce_eig_n54_chol_smem_bytes() + L.geneig_reduce_eig_n54_heev_smem_bytes() == \
         L.geneig_reduce_eig_n54_shared_mem_bytes()
 

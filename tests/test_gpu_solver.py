@@ -131,7 +131,7 @@ def test_bpb_that_does_not_fit_raises(geneig, device, cache_dir):
     """An explicit pairs-per-workgroup beyond the LDS budget is refused, as the
     reference refuses an explicit BPB that does not fit (nvrtc_solver.cpp:1021-1029)."""
     with pytest.raises(geneig.GeneigError, match="LDS"):
-        geneig.GeneigSolver(54, batches_per_block_request=4)
+        geneig.GeneigSolver(54, batches_per_block_request=8)   # 8 x 26.3 KB > 160 KB
 
 
 def test_runtime_matches_static_n54(geneig, oracle, device, cache_dir):
