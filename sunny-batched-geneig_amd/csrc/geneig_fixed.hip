@@ -52,7 +52,8 @@ __global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigne
 
 __global__ void __launch_bounds__(64 * C54::R, GENEIG_K2B_WAVES_PER_EU) k2b_kernel(const int* __restrict__ info,
                                                                                  unsigned char* ws, int batch) {
-    geneig::k2b_batched<kN, double>(info, ws, batch);
+    __shared__ geneig::cplx buf[geneig::k2b_buf_entries<kN, double>()];
+    geneig::k2b_batched<kN, double>(info, ws, batch, buf);
 }
 
 __global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2x_kernel(int* __restrict__ info, unsigned char* ws, int batch) {

@@ -75,7 +75,8 @@ __global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsign
 
 __global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2bf_kernel(const int* __restrict__ info,
                                                                                  unsigned char* ws, int batch) {
-    geneig::k2b_batched<kN, float>(info, ws, batch);
+    __shared__ cplxf buf[geneig::k2b_buf_entries<kN, float>()];
+    geneig::k2b_batched<kN, float>(info, ws, batch, buf);
 }
 
 __global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2xf_kernel(int* __restrict__ info,
@@ -120,7 +121,8 @@ __global__ void __launch_bounds__(64) k2al_kernel(int* __restrict__ info, unsign
 __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2B_WAVES_PER_EU) k2bl_kernel(const int* __restrict__ info,
                                                                                  unsigned char* ws, int count,
                                                                                  const int* __restrict__ list) {
-    geneig::k2b_batched<kN, double>(info, ws, count, list);
+    __shared__ cplx buf[geneig::k2b_buf_entries<kN, double>()];
+    geneig::k2b_batched<kN, double>(info, ws, count, buf, list);
 }
 
 __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2xl_kernel(int* __restrict__ info,
