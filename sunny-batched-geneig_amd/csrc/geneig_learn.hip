@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(kThreads) jacobi_kernel(const float* __restric
 constexpr unsigned kLossLds = 6u * kNN * sizeof(float2);   // 142,560 B
 
 // pipeline kernels of the learned form (one pair per workgroup)
-__global__ void __launch_bounds__(64) chol64_kernel(const cplx* __restrict__ B, double* __restrict__ W,
+__global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                      cplx* __restrict__ U, int* __restrict__ info, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
