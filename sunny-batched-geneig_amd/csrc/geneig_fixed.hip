@@ -45,9 +45,9 @@ constexpr unsigned kK1Lds = geneig::k1_lds_bytes<kN, double>();
 constexpr unsigned kWorkLds = kK1Lds > kK3Lds ? kK1Lds : kK3Lds;
 constexpr unsigned kVecLds = sizeof(geneig::cplx) * C54::VEC;
 
-__global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+__global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch, int cap) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2a_batched<kN, double>(info, ws, batch, reinterpret_cast<double*>(smem));
+    geneig::k2a_batched<kN, double>(info, ws, batch, reinterpret_cast<double*>(smem), cap);
 }
 
 __global__ void __launch_bounds__(64 * C54::R, GENEIG_K2B_WAVES_PER_EU) k2b_kernel(const int* __restrict__ info,
@@ -119,7 +119,8 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
                            b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
         if (!eig) continue;
-        hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, stream, info + off, w, P);
+        hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, stream, info + off, w, P,
+                           geneig_host::ql_log_entries());
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream, info + off, w, P);
         RETURN_IF(hipGetLastError());

@@ -68,9 +68,9 @@ constexpr unsigned kK1LdsF = geneig::k1_lds_bytes<kN, float>();
 constexpr unsigned kK1LdsD = geneig::k1_lds_bytes<kN, double>();
 constexpr unsigned kK3LdsD = geneig::k3_lds_bytes<kN, double>();
 
-__global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+__global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch, int cap) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2a_batched<kN, float>(info, ws, batch, reinterpret_cast<float*>(smem));
+    geneig::k2a_batched<kN, float>(info, ws, batch, reinterpret_cast<float*>(smem), cap);
 }
 
 __global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2bf_kernel(const int* __restrict__ info,
@@ -113,9 +113,9 @@ __global__ void __launch_bounds__(64) k1l_kernel(const cplx* __restrict__ A, con
 }
 
 __global__ void __launch_bounds__(64) k2al_kernel(int* __restrict__ info, unsigned char* ws, int count,
-                                                   const int* __restrict__ list) {
+                                                   const int* __restrict__ list, int cap) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2a_batched<kN, double>(info, ws, count, reinterpret_cast<double*>(smem), list);
+    geneig::k2a_batched<kN, double>(info, ws, count, reinterpret_cast<double*>(smem), cap, list);
 }
 
 __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2B_WAVES_PER_EU) k2bl_kernel(const int* __restrict__ info,
@@ -199,7 +199,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
                            W + (size_t)kN * off, U + mat * off, info + off, solver, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, stream, info + off,
-                           solver, P);
+                           solver, P, geneig_host::ql_log_entries());
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2bf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
         RETURN_IF(hipGetLastError());
@@ -224,7 +224,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
                            fb_list + off);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2al_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsD, stream, info, solver,
-                           P, fb_list + off);
+                           P, fb_list + off, geneig_host::ql_log_entries());
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2bl_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
                            fb_list + off);

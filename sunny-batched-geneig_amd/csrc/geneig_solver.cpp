@@ -448,7 +448,9 @@ struct geneig_solver {
             const unsigned g13 = (unsigned)((P + b - 1) / b);
             HIP_TRY(hipModuleLaunchKernel(m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1, nullptr));
             void* a2[] = {&inf, &ws, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.fn2a, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2,
+            int cap = geneig_host::ql_log_entries();
+            void* a2a[] = {&inf, &ws, &P, &cap};
+            HIP_TRY(hipModuleLaunchKernel(m.fn2a, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a,
                                           nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.fn2b, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
@@ -493,7 +495,9 @@ struct geneig_solver {
             void* a1[] = {&h, &sp, &w, &u, &inf, &solver, &P};
             HIP_TRY(hipModuleLaunchKernel(m.f1f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a1, nullptr));
             void* a2[] = {&inf, &solver, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.f2af, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.f_k2a_lds, stream, a2,
+            int cap = geneig_host::ql_log_entries();
+            void* a2a[] = {&inf, &solver, &P, &cap};
+            HIP_TRY(hipModuleLaunchKernel(m.f2af, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.f_k2a_lds, stream, a2a,
                                           nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.f2bf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.f2xf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
@@ -514,7 +518,9 @@ struct geneig_solver {
             void* a1[] = {&H, &S, &W, &U, &info, &solver, &P, &lst};
             HIP_TRY(hipModuleLaunchKernel(m.f1l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a1, nullptr));
             void* a2[] = {&info, &solver, &P, &lst};
-            HIP_TRY(hipModuleLaunchKernel(m.f2al, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2,
+            int cap = geneig_host::ql_log_entries();
+            void* a2a[] = {&info, &solver, &P, &lst, &cap};
+            HIP_TRY(hipModuleLaunchKernel(m.f2al, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a,
                                           nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.f2bl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
             HIP_TRY(hipModuleLaunchKernel(m.f2xl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));

@@ -2,6 +2,7 @@
 #include "geneig_ws.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -38,6 +39,13 @@ int chunk_pairs(int batch, unsigned long long pair_bytes) {
     if (batch <= 0) return 0;
     const unsigned long long c = std::max<unsigned long long>(1, budget_bytes() / pair_bytes);
     return (int)std::min<unsigned long long>((unsigned long long)batch, c);
+}
+
+int ql_log_entries() {
+    const char* e = std::getenv("GENEIG_QL_LOG_ENTRIES");
+    if (!e || !*e) return INT_MAX;
+    const long v = std::atol(e);
+    return v > 0 && v < INT_MAX ? (int)v : INT_MAX;
 }
 
 hipError_t workspace(hipStream_t stream, size_t bytes, void** out) {

@@ -22,4 +22,10 @@ int chunk_pairs(int batch, unsigned long long pair_bytes);
 // Workspace of at least `bytes` for `stream` on the current device.
 hipError_t workspace(hipStream_t stream, size_t bytes, void** out);
 
+// Entries the QL rotation log may take per pair: GENEIG_QL_LOG_ENTRIES when set
+// (read at every launch; a small value sends every pair through the one-pair
+// QL that handles an overflowed log, which is how that path is tested), else
+// no limit below the compiled capacity.
+int ql_log_entries();
+
 }  // namespace geneig_host

@@ -246,3 +246,40 @@ def test_deterministic_bitwise(geneig, oracle, device):
         torch.cuda.synchronize()
         outs.append((to_host(W).tobytes(), to_host(U).tobytes()))
     assert outs[0] == outs[1]
+
+
+def test_ql_log_overflow_path(geneig, oracle, device, monkeypatch):
+    """With the QL rotation log capped below one QL iteration
+    (GENEIG_QL_LOG_ENTRIES, read at every launch) every pair is solved by the
+    one-pair QL that handles an overflowed log: the static FP64 launch, the
+    precision form and a runtime module still meet the FP64 bar, and the
+    eigenvalues agree with the logged path to rounding."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 96, seed=5)
+    Ah, Bh = to_host(A), to_host(B)
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    Ag, Bg = geneig.gen_generic(40, 96, seed=6)
+    Agh, Bgh = to_host(Ag), to_host(Bg)
+    Wgo, Xgo, _ = oracle.zhegv(Agh, Bgh)
+    s40 = geneig.GeneigSolver(40)
+    res = {}
+    for cap in (None, "16"):
+        if cap is not None:
+            monkeypatch.setenv("GENEIG_QL_LOG_ENTRIES", cap)
+        W, U, info = geneig.empty_outputs(54, 96, device)
+        geneig.full_n54_batched(A, B, W, U, info)
+        Wm, Um, infom = geneig.empty_outputs(54, 96, device)
+        fb = geneig.mixed_n54_batched(A, B, Wm, Um, infom)
+        Wg, Ug, infog = geneig.empty_outputs(40, 96, device)
+        s40.launch(Ag, Bg, Wg, Ug, infog)
+        torch.cuda.synchronize()
+        tag = "log cap %s" % cap
+        for i in (info, infom, infog):
+            assert int(i.abs().max()) == 0, tag
+        check_pairs(oracle, Ah, Bh, to_host(W), to_host(U), Wo, Xo, 54, tag + " FP64")
+        check_pairs(oracle, Ah, Bh, to_host(Wm), to_host(Um), Wo, Xo, 54, tag + " precision (fallback %d)" % fb)
+        check_pairs(oracle, Agh, Bgh, to_host(Wg), to_host(Ug), Wgo, Xgo, 40, tag + " runtime n=40")
+        res[cap] = (to_host(W), to_host(Wm), to_host(Wg))
+    for a, b in zip(res[None], res["16"]):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
