@@ -54,6 +54,23 @@ int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void*
                                  double* d_W, void* d_U, int* d_info, int total_matrices,
                                  int chunk_size, int num_streams, const hipStream_t* streams);
 
+/* launch_chunked for host-resident pairs: chunks of chunk_size pairs go
+ * round-robin over the streams, and on its stream each chunk is copied in
+ * (H, S), solved and copied out (W, U, info), so with two or more streams the
+ * copies of one chunk overlap the solve of another.  The reference's
+ * launch_chunked (nvrtc_solver.cpp:1078-1107) takes device pointers only and
+ * its copy overlap was deferred (docs/PHASE3_NVRTC.md:76-89); this entry has
+ * no counterpart there.  Host buffers should be pinned (hipHostMalloc):
+ * pageable ones give the same results without overlap.  The solver keeps one
+ * device staging slot of chunk_size pairs per stream (allocated on first use,
+ * grown on demand, freed by geneig_solver_destroy).  Returns once everything
+ * is enqueued; the host outputs are complete after every stream is
+ * synchronized.  On a precision-form solver each chunk's launch waits for its
+ * own fallback count, which serializes the chunks. */
+int geneig_solver_launch_chunked_host(geneig_solver_t s, const void* h_H, const void* h_S,
+                                      double* h_W, void* h_U, int* h_info, int total_matrices,
+                                      int chunk_size, int num_streams, const hipStream_t* streams);
+
 /* Allocates, ahead of time, the device workspace a launch of batch_size
  * pairs on `stream` will use, so that the launch itself allocates nothing
  * and can be captured into a graph.  No counterpart in the reference (whose

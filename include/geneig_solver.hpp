@@ -63,6 +63,14 @@ public:
                                            num_streams, streams.data()),
               "GeneigSolver::launch_chunked");
     }
+    // Host-resident pairs (pinned for copy overlap): copy in, solve, copy out
+    // per chunk on its stream; outputs complete once the streams are synchronized.
+    void launch_chunked_host(const void* h_H, const void* h_S, double* h_W, void* h_U, int* h_info,
+                             int total_matrices, int chunk_size, const std::vector<hipStream_t>& streams) {
+        check(geneig_solver_launch_chunked_host(h_, h_H, h_S, h_W, h_U, h_info, total_matrices, chunk_size,
+                                                (int)streams.size(), streams.data()),
+              "GeneigSolver::launch_chunked_host");
+    }
 
     int matrix_size() const { return geneig_solver_matrix_size(h_); }
     int batches_per_block() const { return geneig_solver_batches_per_block(h_); }

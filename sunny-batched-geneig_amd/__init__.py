@@ -75,6 +75,7 @@ def lib():
         "geneig_solver_destroy": (None, [P]),
         "geneig_solver_launch": (I, [P, P, P, P, P, P, I, P]),
         "geneig_solver_launch_chunked": (I, [P, P, P, P, P, P, I, I, I, P]),
+        "geneig_solver_launch_chunked_host": (I, [P, P, P, P, P, P, I, I, I, P]),
         "geneig_solver_matrix_size": (I, [P]),
         "geneig_solver_batches_per_block": (I, [P]),
         "geneig_solver_block_dim": (None, [P, P, P, P]),
@@ -164,17 +165,28 @@ def _require(t, dtype, name):
         raise GeneigError("%s must be contiguous" % name)
 
 
-def _check_batch(n, batch, A, B, W, U, info):
+def _require_host(t, dtype, name):
     torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.is_cuda:
+        raise GeneigError("%s must be a host tensor" % name)
+    if t.dtype != dtype:
+        raise GeneigError("%s must be %s, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise GeneigError("%s must be contiguous" % name)
+
+
+def _check_batch(n, batch, A, B, W, U, info, host=False):
+    torch = _torch()
+    req = _require_host if host else _require
     for t, nm in ((A, "A"), (B, "B"), (U, "U")):
-        _require(t, torch.complex128, nm)
+        req(t, torch.complex128, nm)
         if t.numel() < batch * n * n:
             raise GeneigError("%s holds %d elements, batch*n*n = %d" % (nm, t.numel(), batch * n * n))
     if W is not None:
-        _require(W, torch.float64, "W")
+        req(W, torch.float64, "W")
         if W.numel() < batch * n:
             raise GeneigError("W holds %d elements, batch*n = %d" % (W.numel(), batch * n))
-    _require(info, torch.int32, "info")
+    req(info, torch.int32, "info")
     if info.numel() < batch:
         raise GeneigError("info holds %d elements, batch = %d" % (info.numel(), batch))
 
@@ -282,6 +294,16 @@ class GeneigSolver:
         _check(lib().geneig_solver_launch_chunked(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
                                                   int(total), int(chunk_size), len(streams),
                                                   C.cast(arr, C.c_void_p)), "GeneigSolver.launch_chunked")
+
+    def launch_chunked_host(self, H, S, W, U, info, total, chunk_size, streams):
+        """launch_chunked over host tensors (pinned for copy overlap): per chunk, on its
+        stream, H and S are copied in, solved and W, U, info copied out.  Outputs are
+        complete once every stream is synchronized."""
+        _check_batch(self.matrix_size(), total, H, S, W, U, info, host=True)
+        arr = (C.c_void_p * len(streams))(*[s.cuda_stream for s in streams])
+        _check(lib().geneig_solver_launch_chunked_host(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
+                                                       int(total), int(chunk_size), len(streams),
+                                                       C.cast(arr, C.c_void_p)), "GeneigSolver.launch_chunked_host")
 
     def matrix_size(self):
         return lib().geneig_solver_matrix_size(self._h)

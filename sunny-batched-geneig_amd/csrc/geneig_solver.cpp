@@ -352,6 +352,40 @@ struct geneig_solver {
     bool was_autotuned = false, sweep_ran = false;
     const char* layer = "compile";
     double acquire_ms = 0.0;
+    // Device staging of launch_chunked_host, one slot per stream: work
+    // queued on a stream is ordered, so a slot is reused only after the
+    // copies of its previous chunk.  Grown on demand, freed with the solver.
+    struct Staging {
+        void* base = nullptr;
+        size_t bytes = 0;
+    };
+    std::mutex stage_mu;
+    std::map<hipStream_t, Staging> stage;
+
+    ~geneig_solver() {
+        if (stage.empty()) return;
+        (void)hipSetDevice(device);
+        for (auto& kv : stage)
+            if (kv.second.base) (void)hipFree(kv.second.base);   // waits for the device
+    }
+
+    // A staging slot of at least `bytes` for `stream`.
+    int staging(hipStream_t stream, size_t bytes, void** out) {
+        std::lock_guard<std::mutex> lk(stage_mu);
+        Staging& st = stage[stream];
+        if (st.bytes < bytes) {
+            if (st.base) {
+                HIP_TRY(hipStreamSynchronize(stream));   // the old slot may still be in use on this stream
+                (void)hipFree(st.base);
+                st.base = nullptr;
+                st.bytes = 0;
+            }
+            HIP_TRY(hipMalloc(&st.base, bytes));
+            st.bytes = bytes;
+        }
+        *out = st.base;
+        return GENEIG_E_OK;
+    }
 
     // Module acquisition through the cache layers (nvrtc_solver.cpp:522-597).
     int acquire(int want_bpb, int want_variant, int cache_mode, std::shared_ptr<Module>& out,
@@ -835,6 +869,40 @@ int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void*
                                       d_W + (size_t)s->n * off, (char*)d_U + mat * off, d_info + off, bs,
                                       streams[ci % nstreams]);
         if (rc) return rc;
+    }
+    return GENEIG_E_OK;
+}
+
+int geneig_solver_launch_chunked_host(geneig_solver_t s, const void* h_H, const void* h_S, double* h_W,
+                                      void* h_U, int* h_info, int total, int chunk, int nstreams,
+                                      const hipStream_t* streams) {
+    if (!s) return fail(GENEIG_E_ARG, "null solver");
+    if (total <= 0) return GENEIG_E_OK;
+    if (!h_H || !h_S || !h_W || !h_U || !h_info) return fail(GENEIG_E_ARG, "launch_chunked_host: null host buffer");
+    if (chunk <= 0) return fail(GENEIG_E_ARG, "launch_chunked_host: chunk_size must be > 0");
+    if (nstreams <= 0 || !streams) return fail(GENEIG_E_ARG, "launch_chunked_host: need num_streams > 0 streams");
+    const size_t n = (size_t)s->n, mat = n * n * 16;
+    const size_t cp = (size_t)std::min(chunk, total);
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t oS = up(mat * cp), oU = oS + up(mat * cp), oW = oU + up(mat * cp), oI = oW + up(8 * n * cp);
+    const size_t bytes = oI + up(4 * cp);
+    int ci = 0;
+    for (int off = 0; off < total; off += chunk, ++ci) {
+        const int bs = std::min(chunk, total - off);
+        const hipStream_t st = streams[ci % nstreams];
+        void* base = nullptr;
+        int rc = s->staging(st, bytes, &base);
+        if (rc) return rc;
+        char* b = static_cast<char*>(base);
+        const size_t m = mat * (size_t)bs;
+        HIP_TRY(hipMemcpyAsync(b, (const char*)h_H + mat * off, m, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(b + oS, (const char*)h_S + mat * off, m, hipMemcpyHostToDevice, st));
+        rc = geneig_solver_launch(s, b, b + oS, reinterpret_cast<double*>(b + oW), b + oU,
+                                  reinterpret_cast<int*>(b + oI), bs, st);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(h_W + n * off, b + oW, 8 * n * bs, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync((char*)h_U + mat * off, b + oU, m, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(h_info + off, b + oI, 4 * (size_t)bs, hipMemcpyDeviceToHost, st));
     }
     return GENEIG_E_OK;
 }

@@ -6,7 +6,8 @@ reference's tests:
   tests/test_autotune_n54.cpp       sweep runs once, then the cache is hit
   tests/test_bpb_sweep_n54.cpp      explicit pairs-per-workgroup sweep, same results
   tests/test_nvrtc_multiN.cpp       every fixture size through the runtime path
-and NvrtcGeneigSolver::launch_chunked (src/nvrtc/nvrtc_solver.cpp:1078-1107).
+and NvrtcGeneigSolver::launch_chunked (src/nvrtc/nvrtc_solver.cpp:1078-1107),
+plus its host-buffer variant with copy overlap (no reference counterpart).
 """
 import os
 
@@ -170,6 +171,34 @@ def test_launch_chunked_streams(geneig, oracle, device, cache_dir):
     s.launch_chunked(A, B, W, U, info, total=100, chunk_size=17, streams=streams)
     torch.cuda.synchronize()
     assert np.array_equal(W.cpu().numpy(), W0) and np.array_equal(U.cpu().numpy(), U0)
+
+
+@pytest.mark.parametrize("pinned", [True, False])
+def test_launch_chunked_host_matches_device(geneig, oracle, device, cache_dir, pinned):
+    """Host-resident pairs through launch_chunked_host (copy in, solve, copy out per
+    chunk on its stream) give the device launch's outputs bit for bit; pageable
+    host buffers are slower but equal."""
+    import torch
+
+    n, total = 32, 300
+    A, B = geneig.gen_generic(n, total, seed=4)
+    s = geneig.GeneigSolver(n)
+    W0, U0, i0 = run(geneig, s, A, B, device)
+    Ah = A.cpu().pin_memory() if pinned else A.cpu()
+    Bh = B.cpu().pin_memory() if pinned else B.cpu()
+    W = torch.full((total, n), float("nan"), dtype=torch.float64, pin_memory=pinned)
+    U = torch.full((total, n, n), float("nan"), dtype=torch.complex128, pin_memory=pinned)
+    info = torch.full((total,), -1, dtype=torch.int32, pin_memory=pinned)
+    streams = geneig.make_streams(3)
+    torch.cuda.synchronize()
+    for rep in range(2):   # the second call reuses the per-stream staging slots
+        s.launch_chunked_host(Ah, Bh, W, U, info, total=total, chunk_size=41, streams=streams)
+        for st in streams:
+            st.synchronize()
+        assert np.array_equal(W.numpy(), W0) and np.array_equal(U.numpy(), U0), rep
+        assert np.array_equal(info.numpy(), i0) and (info.numpy() == 0).all()
+    with pytest.raises(geneig.GeneigError, match="host tensor"):
+        s.launch_chunked_host(A, B, W, U, info, total=total, chunk_size=41, streams=streams)
 
 
 def test_workspace_chunking_small_budget(geneig, oracle, device):
