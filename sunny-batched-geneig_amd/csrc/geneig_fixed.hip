@@ -35,6 +35,17 @@ __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __res
     geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, smem, vecs);
 }
 
+// The FP64 Cholesky alone (two waves per SIMD); the batched FP64 solve runs it
+// ahead of K1, which then starts from the stashed factor (MODE_MIXED).
+__global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(const geneig::cplx* __restrict__ B,
+                                                                           double* __restrict__ W,
+                                                                           geneig::cplx* __restrict__ U,
+                                                                           int* __restrict__ info, int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ geneig::cplx vecs[C54::VEC];
+    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, smem, vecs);
+}
+
 // K2 in three kernels: the QL chases of 64 pairs per wave with the rotations
 // logged (K2a), their replay into Z (K2b), and the one-pair-per-workgroup QL
 // for the rare pair whose log overflowed (K2x).
@@ -94,9 +105,15 @@ hipError_t set_lds_once(const void* kernel, unsigned bytes) {
 // MODE_FINISH takes M in A and L in B.
 template <int MODE>
 hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int batch, hipStream_t stream) {
-    auto k1 = k1_kernel<MODE>;
+    // the batched FP64 solve: the Cholesky as its own kernel (two waves per
+    // SIMD), then K1 from the stashed factor; results are bit-identical (the
+    // same operations per entry, only the update chunking differs)
+    constexpr bool split_chol = (MODE == geneig::MODE_FULL);
+    auto k1 = k1_kernel<split_chol ? geneig::MODE_MIXED : MODE>;
     auto k3 = k3_kernel<MODE>;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), kK1Lds * kBPB));
+    if (split_chol)
+        RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
 
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), kK3Lds * kBPB));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
@@ -115,6 +132,11 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
         const int P = std::min(chunk, batch - off);
         const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
         unsigned char* w = static_cast<unsigned char*>(ws);
+        if (split_chol) {
+            hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), stream,
+                               b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, P);
+            RETURN_IF(hipGetLastError());
+        }
         hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, stream, a + mat * off,
                            b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
