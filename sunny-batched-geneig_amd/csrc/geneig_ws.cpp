@@ -23,10 +23,11 @@ std::map<std::pair<int, hipStream_t>, Buf> g_bufs;
 size_t budget_bytes() {
     static const size_t b = [] {
         const char* e = std::getenv("GENEIG_WORKSPACE_MB");
-        // 8 GiB by default: the QL stage runs 64 pairs per wave as serial
-        // chains, so it needs tens of thousands of pairs per dispatch to
-        // occupy the chip, and 8 GiB is under 3% of an MI355X's HBM.
-        long mb = e ? std::atol(e) : 8192;
+        // 16 GiB by default: the QL stage runs 64 pairs per wave as serial
+        // chains, so it needs as many pairs per dispatch as possible to hide
+        // their latency (131,072 FP64 pairs at n = 54 fit one chunk), and
+        // 16 GiB is under 6% of an MI355X's HBM.
+        long mb = e ? std::atol(e) : 16384;
         if (mb < 16) mb = 16;
         return (size_t)mb << 20;
     }();

@@ -4,7 +4,7 @@
 // the rotation product in registers) and K3 (sort, Y = Q Z, back-transform);
 // they hand d, e, Q and Z over in a workspace of Ws<n>::PAIR_BYTES per pair.
 // A batch is processed in chunks so the workspace stays bounded (default
-// 8 GiB, GENEIG_WORKSPACE_MB overrides).  Workspaces are kept per (device,
+// 16 GiB, GENEIG_WORKSPACE_MB overrides).  Workspaces are kept per (device,
 // stream), allocated on first use and grown when a larger chunk is needed,
 // so launches on different streams never share one; launches that overlap on
 // the same stream are ordered by the stream itself.
