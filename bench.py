@@ -280,6 +280,61 @@ def bench_learned(args, geneig, torch, dev):
     print(json.dumps(line), flush=True)
 
 
+def bench_generic(args, geneig, torch, dev):
+    """Generic pairs at any n through the runtime (hipRTC) solvers: the FP64 form
+    and the precision form on the same batch, device-resident, HIP-event timed."""
+    n, batch = args.n, args.batch
+    A, B = geneig.gen_generic(n, batch, seed=args.seed)
+    W, U, info = geneig.empty_outputs(n, batch, dev)
+    sp, sm = geneig.GeneigSolver(n), geneig.GeneigMixedSolver(n)
+    iters = torch.zeros(batch, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / args.steps
+
+    plain_s = timed(lambda: sp.launch(A, B, W, U, info, stream=stream))
+    Wp = W.clone()
+    sel = torch.arange(0, batch, max(1, batch // args.check), device=dev)[: args.check]
+    acc_p = accuracy(A.index_select(0, sel).cpu().numpy(), B.index_select(0, sel).cpu().numpy(),
+                     W.index_select(0, sel).cpu().numpy(), U.index_select(0, sel).cpu().numpy())
+    fb = [0]
+
+    def mixed():
+        fb[0] = sm.launch_mixed(A, B, W, U, info, iters=iters, stream=stream)
+
+    mixed_s = timed(mixed)
+    acc_m = accuracy(A.index_select(0, sel).cpu().numpy(), B.index_select(0, sel).cpu().numpy(),
+                     W.index_select(0, sel).cpu().numpy(), U.index_select(0, sel).cpu().numpy())
+    v, c = np.unique(iters.cpu().numpy(), return_counts=True)
+    log("n=%d: FP64 %.1f ms, precision %.1f ms" % (n, plain_s * 1e3, mixed_s * 1e3))
+    line = {
+        "metric": "n=%d generic geneig solves/s, precision vs plain FP64 form (runtime solvers)" % n,
+        "value": batch / mixed_s, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": mixed_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 bulk + f64 (Cholesky, refinement)", "data": "synthetic (device-generated generic pairs, seed %d)"
+        % args.seed,
+        "config": {"workload": "generic pairs A = (M + M^H)/(2 sqrt n), B = G G^H/n + I, n=%d, %d pairs" % (n, batch),
+                   "n": n, "global_batch": batch, "parallelism": "none (1 GPU)"},
+        "accuracy": acc_m, "fallback_pairs": int(fb[0]),
+        "refine_steps_histogram": {str(int(a)): int(b) for a, b in zip(v, c)},
+        "plain": {"value": batch / plain_s, "ms_per_step": plain_s * 1e3, "accuracy": acc_p},
+        "speedup_vs_plain": plain_s / mixed_s,
+        "roofline": {"bound": "mfma", "achieved": flops_per_pair(n) * batch / mixed_s / 1e12,
+                     "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": flops_per_pair(n) * batch / mixed_s / FP64_PEAK, "traffic": None,
+                     "kernel": "one precision-form launch (wall time per step)"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_host_chunked(args, geneig, torch, dev):
     """n = 54 spin-wave pairs held in pinned host memory: launch_chunked_host copies
     each chunk in, solves it and copies it out on its stream, so with several
@@ -344,13 +399,15 @@ def main():
     ap.add_argument("--check", type=int, default=256, help="pairs checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["spinwave54", "mixed-sizes", "learned", "host-chunked"],
+    ap.add_argument("--workload", choices=["spinwave54", "mixed-sizes", "learned", "host-chunked", "generic"],
                     default="spinwave54",
                     help="spinwave54: the headline metric; mixed-sizes: 200,000 generic pairs over "
                          "n in {8,...,96}, one graph launch vs one launch per size; learned: train the "
                          "warm-start network on 32,768 solver-produced C, evaluate on 262,144 unseen q; "
                          "host-chunked: the n=54 pairs from pinned host memory through launch_chunked_host "
-                         "(PCIe-inclusive, 1/2/4 streams)")
+                         "(PCIe-inclusive, 1/2/4 streams); generic: generic pairs at --n through the runtime "
+                         "FP64 and precision solvers")
+    ap.add_argument("--n", type=int, default=96, help="matrix size of --workload generic")
     ap.add_argument("--host-chunk", type=int, default=8192, help="chunk size of --workload host-chunked")
     ap.add_argument("--train-epochs", type=int, default=40)
     ap.add_argument("--eval-batch", type=int, default=262144)
@@ -387,6 +444,9 @@ def main():
         return
     if args.workload == "host-chunked":
         bench_host_chunked(args, geneig, torch, dev)
+        return
+    if args.workload == "generic":
+        bench_generic(args, geneig, torch, dev)
         return
     n, nsub, batch = 54, 27, args.batch
     grid = (64, 64, 32)
