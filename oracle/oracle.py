@@ -281,6 +281,32 @@ def compare(A, B, W, X, Wr, Xr=None, degen_gap=1e-6, nthreads=0):
     return m.as_dict()
 
 
+def rayleigh_ld(A, B, X):
+    """Rayleigh quotients x_j^H A x_j / x_j^H B x_j of every column of X, per
+    pair, evaluated in long double (64-bit significand) and returned as float64
+    (batch x n).  A checker with no reference counterpart: the quotient is
+    stationary at an eigenvector, so its error is second order in the error of
+    x_j, and with the oracle's eigenvectors it gives eigenvalues much closer to
+    exact than any FP64 eigenvalue when B is ill-conditioned (there the FP64
+    eigenvalues, the oracle's included, carry errors ~ eps cond(B))."""
+    A, B, X = _c128(A), _c128(B), _c128(X)
+    out = np.empty(X.shape[:2], np.float64)
+    for p in range(A.shape[0]):
+        Am = A[p].T.astype(np.clongdouble)        # pair p stored as its transpose
+        Bm = B[p].T.astype(np.clongdouble)
+        Xm = X[p].T.astype(np.clongdouble)        # columns = eigenvectors
+        num = (Xm.conj() * (Am @ Xm)).sum(axis=0).real
+        den = (Xm.conj() * (Bm @ Xm)).sum(axis=0).real
+        out[p] = (num / den).astype(np.float64)
+    return out
+
+
+def eig_err_scaled(W, Wr):
+    """max_i |W_i - Wr_i| / max(|Wr_i|, 1) over a batch (the oracle's eig_rel_scaled)."""
+    W, Wr = np.asarray(W, np.float64), np.asarray(Wr, np.float64)
+    return float((np.abs(W - Wr) / np.maximum(np.abs(Wr), 1.0)).max())
+
+
 def residuals(A, B, W, X):
     """(resid, orth, resid_raw, orth_raw) of one pair."""
     A, B, X = _c128(A), _c128(B), _c128(X)

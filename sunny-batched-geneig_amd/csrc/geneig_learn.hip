@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3j_kernel(const 
     __shared__ cplxf vecs[CF::VEC];
     geneig::k3_batched<kN, 1, geneig::MODE_JACOBI, float>(B, W, U, info, ws, batch, smem, vecs);
 }
-__global__ void __launch_bounds__(64 * RC::NT, 2) refine_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
+__global__ void __launch_bounds__(64 * RC::NT, geneig::refine_waves_per_eu<kN>()) refine_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                                  double* __restrict__ W, cplx* __restrict__ U,
                                                                  const int* __restrict__ info, int* __restrict__ fb_list,
                                                                  int* __restrict__ fb_count, int* __restrict__ iters,

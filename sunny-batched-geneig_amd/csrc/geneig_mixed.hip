@@ -25,10 +25,8 @@
 #ifndef GENEIG_K2F_WAVES_PER_EU
 #define GENEIG_K2F_WAVES_PER_EU 4
 #endif
-// Refinement: 2 waves per SIMD keeps every accumulator in VGPRs without
-// spilling (two pairs per CU).
 #ifndef GENEIG_REFINE_WAVES_PER_EU
-#define GENEIG_REFINE_WAVES_PER_EU 2
+#define GENEIG_REFINE_WAVES_PER_EU (geneig::refine_waves_per_eu<54>())
 #endif
 
 namespace {

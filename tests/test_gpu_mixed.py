@@ -87,7 +87,13 @@ def test_mixed_condition_sweep(geneig, oracle, device, cond):
     """cond(B) from 1e2 to 1e8: the refinement converges or the pair is re-solved
     in FP64.  Every metric against the oracle meets the FP64 bar or, where
     cond(B) puts the FP64 path itself above that bar, stays within twice the
-    FP64 path's figure on the same pairs."""
+    FP64 path's figure on the same pairs.  The eigenvalues at cond(B) >= 1e6
+    are the exception: there the oracle's own FP64 eigenvalues are off by
+    ~ eps cond(B) (2-3e-12 at 1e6), as large as the errors measured, so both
+    forms are checked against long-double Rayleigh quotients of the oracle's
+    eigenvectors (second order in their error) with the first-order
+    perturbation bound eps cond(B) of a generalized eigenvalue under a
+    relative perturbation of B."""
     import torch
 
     from test_gpu_parity import EPS, TOL_EIG, TOL_PHASE
@@ -105,7 +111,15 @@ def test_mixed_condition_sweep(geneig, oracle, device, cond):
     bars = {"eig_rel_scaled": TOL_EIG, "phase_err": TOL_PHASE, "resid": 50 * 54 * EPS, "orth": 50 * 54 * EPS}
     for k, bar in bars.items():
         print("  %-15s mixed %.2e  fp64 path %.2e  bar %.1e" % (k, mm[k], mp[k], bar), flush=True)
+        if k == "eig_rel_scaled" and cond >= 1e6:
+            continue
         assert mm[k] <= max(bar, 2.0 * mp[k]), k
+    if cond >= 1e6:
+        R = oracle.rayleigh_ld(A, B, Xo)
+        em, ep, eo = (oracle.eig_err_scaled(w, R) for w in (W, to_host(W2), Wo))
+        print("  eig vs long-double Rayleigh quotients: mixed %.2e  fp64 path %.2e  oracle %.2e  bar %.1e"
+              % (em, ep, eo, EPS * cond), flush=True)
+        assert em <= EPS * cond and ep <= EPS * cond
     if cond <= 1e4:
         assert fb == 0
 

@@ -294,3 +294,28 @@ def test_fnv1a64_known_answers(oracle):
     assert oracle.fnv1a64(b"") == 0xcbf29ce484222325
     assert oracle.fnv1a64(b"a") == 0xaf63dc4c8601ec8c
     assert oracle.fnv1a64(b"foobar") == 0x85944171f73967e8
+
+
+def test_rayleigh_ld_pinned(oracle):
+    """The long-double Rayleigh quotient used as the sharper reference at large
+    cond(B): exact on a diagonal pair, equal to the eigenvalues for exact
+    eigenvectors, and second order in an eigenvector perturbation (a first-order
+    or transposed formula fails one of these)."""
+    rng = np.random.default_rng(5)
+    n = 12
+    a = rng.standard_normal(n)
+    b = rng.uniform(0.5, 2.0, n)
+    A = np.diag(a).astype(np.complex128)[None]
+    B = np.diag(b).astype(np.complex128)[None]
+    X = (np.eye(n) * (1.0 + 0.5j)).astype(np.complex128)[None]   # scaled unit vectors
+    R = oracle.rayleigh_ld(A, B, X)
+    assert np.allclose(R[0], a / b, rtol=2e-16, atol=0.0)
+    A, B = oracle.gen_generic_cond(24, 3, seed=2, cond=1e2)
+    Wo, Xo, io = oracle.zhegv(A, B)
+    R = oracle.rayleigh_ld(A, B, Xo)
+    assert oracle.eig_err_scaled(R, Wo) < 1e-14
+    # perturb the eigenvectors by ~1e-6: the quotients move by O(1e-12), not O(1e-6)
+    P = Xo + 1e-6 * (rng.standard_normal(Xo.shape) + 1j * rng.standard_normal(Xo.shape)) * np.abs(Xo).max()
+    Rp = oracle.rayleigh_ld(A, B, P)
+    d = oracle.eig_err_scaled(Rp, Wo)
+    assert 1e-15 < d < 1e-9, d
