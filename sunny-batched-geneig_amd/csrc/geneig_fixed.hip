@@ -24,6 +24,14 @@ using W54 = geneig::Ws<kN>;
 // keeps the LDS allocation per workgroup at one pair so three workgroups fit
 // a CU.
 constexpr int kBPB = 1;
+// Concurrent parts of a batched FP64 solve: the one-pair-per-lane QL and the
+// memory-heavy K3 of one part overlap the others' K1.  Three measured fastest
+// at 131,072 pairs (64.0 ms against 64.7 on one stream, 65.0 with two, 64.5
+// with four, where five streams share the four hardware queues).
+#ifndef GENEIG_PLAIN_PARTS
+#define GENEIG_PLAIN_PARTS 3
+#endif
+constexpr int kPlainParts = GENEIG_PLAIN_PARTS;
 
 template <int MODE>
 __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __restrict__ A,
@@ -121,37 +129,55 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
     constexpr bool eig = (MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG ||
                           MODE == geneig::MODE_FINISH);
     const int chunk = geneig_host::chunk_pairs(batch, W54::PAIR_BYTES);
+    // a batched solve that fits one chunk runs as kPlainParts concurrent
+    // parts (each with its own region of the workspace and its own stream)
+    const int parts = split_chol ? geneig_host::part_count(batch, chunk, kPlainParts) : 1;
+    const size_t region = [&](int P) { return ((size_t)P * W54::PAIR_BYTES + 255) / 256 * 256; }((batch + parts - 1) / parts);
     void* ws = nullptr;
     // every mode past the Cholesky uses the workspace (K1 stages W = L^{-1} A there)
-    if (MODE != geneig::MODE_CHOL) RETURN_IF(geneig_host::workspace(stream, (size_t)chunk * W54::PAIR_BYTES, &ws));
+    if (MODE != geneig::MODE_CHOL)
+        RETURN_IF(geneig_host::workspace(stream, parts > 1 ? region * parts : (size_t)chunk * W54::PAIR_BYTES, &ws));
     const size_t mat = (size_t)kN * kN;
     const auto* a = static_cast<const geneig::cplx*>(A);
     const auto* b = static_cast<const geneig::cplx*>(B);
     auto* u = static_cast<geneig::cplx*>(U);
-    for (int off = 0; off < batch; off += chunk) {
-        const int P = std::min(chunk, batch - off);
+    // the stages for pairs [off, off + P) with workspace w on stream st
+    auto pipeline = [&](int off, int P, unsigned char* w, hipStream_t st) -> hipError_t {
         const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
-        unsigned char* w = static_cast<unsigned char*>(ws);
         if (split_chol) {
-            hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), stream,
+            hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), st,
                                b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, P);
             RETURN_IF(hipGetLastError());
         }
-        hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, stream, a + mat * off,
+        hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, st, a + mat * off,
                            b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        if (!eig) continue;
-        hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, stream, info + off, w, P,
+        if (!eig) return hipSuccess;
+        hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, st, info + off, w, P,
                            geneig_host::ql_log_entries());
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream, info + off, w, P);
+        hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, stream, info + off, w, P);
+        hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), kK3Lds * kBPB, stream, b + mat * off,
+        hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), kK3Lds * kBPB, st, b + mat * off,
                            W + (size_t)kN * off, u + mat * off, info + off, w, P);
-        RETURN_IF(hipGetLastError());
+        return hipGetLastError();
+    };
+    if (parts > 1) {
+        hipStream_t subs[geneig_host::kMaxParts];
+        RETURN_IF(geneig_host::fork_streams(stream, parts, subs));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < parts && e == hipSuccess; ++i) {
+            const int off = (int)((long long)batch * i / parts);
+            const int P = (int)((long long)batch * (i + 1) / parts) - off;
+            e = pipeline(off, P, static_cast<unsigned char*>(ws) + region * i, subs[i]);
+        }
+        const hipError_t j = geneig_host::join_streams(stream, parts, subs);
+        return e != hipSuccess ? e : j;
     }
+    for (int off = 0; off < batch; off += chunk)
+        RETURN_IF(pipeline(off, std::min(chunk, batch - off), static_cast<unsigned char*>(ws), stream));
     return hipSuccess;
 }
 

@@ -25,6 +25,14 @@
 #ifndef GENEIG_K2F_WAVES_PER_EU
 #define GENEIG_K2F_WAVES_PER_EU 4
 #endif
+// Concurrent parts of a precision-form launch: one part's matrix-core
+// refinement overlaps another's complex64 stages.  Three measured fastest at
+// 131,072 pairs (55.6 ms against 56.8 on one stream, 56.1 with two).
+#ifndef GENEIG_MIXED_PARTS
+#define GENEIG_MIXED_PARTS 3
+#endif
+constexpr int kMixedParts = GENEIG_MIXED_PARTS;
+
 #ifndef GENEIG_REFINE_WAVES_PER_EU
 #define GENEIG_REFINE_WAVES_PER_EU (geneig::refine_waves_per_eu<54>())
 #endif
@@ -95,9 +103,9 @@ __global__ void __launch_bounds__(64 * RC::NT, GENEIG_REFINE_WAVES_PER_EU) refin
                                                               double* __restrict__ W, cplx* __restrict__ U,
                                                               const int* __restrict__ info, int* __restrict__ fb_list,
                                                               int* __restrict__ fb_count, int* __restrict__ iters,
-                                                              int batch, int maxit, double tol) {
+                                                              int batch, int maxit, double tol, int first) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::refine_batched<kN>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem);
+    geneig::refine_batched<kN>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem, first);
 }
 
 // The FP64 pipeline over a list of pairs (the fallback).
@@ -179,7 +187,12 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     const size_t list_bytes = ((sizeof(int) * ((size_t)batch + 1) + 255) / 256) * 256;
     const int chunk_f = geneig_host::chunk_pairs(batch, WF::PAIR_BYTES);
     const int chunk_d = geneig_host::chunk_pairs(batch, WD::PAIR_BYTES);
-    const size_t solver_bytes = std::max((size_t)chunk_f * WF::PAIR_BYTES, (size_t)chunk_d * WD::PAIR_BYTES);
+    // a batch that fits one complex64 chunk runs as kMixedParts concurrent
+    // parts, each with its own region of the solver area and its own stream
+    const int parts = geneig_host::part_count(batch, chunk_f, kMixedParts);
+    const size_t region = (((size_t)((batch + parts - 1) / parts)) * WF::PAIR_BYTES + 255) / 256 * 256;
+    const size_t solver_bytes = std::max(parts > 1 ? region * parts : (size_t)chunk_f * WF::PAIR_BYTES,
+                                         (size_t)chunk_d * WD::PAIR_BYTES);
     void* ws = nullptr;
     RETURN_IF(geneig_host::workspace(stream, list_bytes + solver_bytes, &ws));
     int* fb_count = static_cast<int*>(ws);
@@ -189,27 +202,51 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
 
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, stream, fb_count);
     RETURN_IF(hipGetLastError());
-    hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B, W, U, info, batch);
-    RETURN_IF(hipGetLastError());
-    for (int off = 0; off < batch; off += chunk_f) {
-        const int P = std::min(chunk_f, batch - off);
-        hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, stream, A + mat * off, B + mat * off,
-                           W + (size_t)kN * off, U + mat * off, info + off, solver, P);
+    // the complex64 stages for pairs [off, off + P) with solver area w on stream st
+    auto bulk = [&](int off, int P, unsigned char* w, hipStream_t st) -> hipError_t {
+        hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, st, A + mat * off, B + mat * off,
+                           W + (size_t)kN * off, U + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, stream, info + off,
-                           solver, P, geneig_host::ql_log_entries());
+        hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, st, info + off, w, P,
+                           geneig_host::ql_log_entries());
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2bf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
+        hipLaunchKernelGGL(k2bf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, st, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, stream, info + off, solver, P);
+        hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, st, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, stream, B + mat * off,
-                           W + (size_t)kN * off, U + mat * off, info + off, solver, P);
+        hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, st, B + mat * off,
+                           W + (size_t)kN * off, U + mat * off, info + off, w, P);
+        return hipGetLastError();
+    };
+    if (parts > 1) {
+        hipStream_t subs[geneig_host::kMaxParts];
+        RETURN_IF(geneig_host::fork_streams(stream, parts, subs));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < parts && e == hipSuccess; ++i) {
+            const int off = (int)((long long)batch * i / parts);
+            const int P = (int)((long long)batch * (i + 1) / parts) - off;
+            hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), subs[i],
+                               B + mat * off, W + (size_t)kN * off, U + mat * off, info + off, P);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = bulk(off, P, solver + region * i, subs[i]);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(refine_kernel, dim3((unsigned)P), dim3(64 * RC::NT), RC::LDS, subs[i], A, B, W, U,
+                                   info, fb_list, fb_count, iters, off + P, maxit, tol, off);
+                e = hipGetLastError();
+            }
+        }
+        const hipError_t j = geneig_host::join_streams(stream, parts, subs);
+        RETURN_IF(e);
+        RETURN_IF(j);
+    } else {
+        hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B,
+                           W, U, info, batch);
+        RETURN_IF(hipGetLastError());
+        for (int off = 0; off < batch; off += chunk_f) RETURN_IF(bulk(off, std::min(chunk_f, batch - off), solver, stream));
+        hipLaunchKernelGGL(refine_kernel, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, W, U, info,
+                           fb_list, fb_count, iters, batch, maxit, tol, 0);
         RETURN_IF(hipGetLastError());
     }
-    hipLaunchKernelGGL(refine_kernel, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, W, U, info,
-                       fb_list, fb_count, iters, batch, maxit, tol);
-    RETURN_IF(hipGetLastError());
 
     int count = 0;
     RETURN_IF(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));

@@ -49,6 +49,92 @@ int ql_log_entries() {
     return v > 0 && v < INT_MAX ? (int)v : INT_MAX;
 }
 
+namespace {
+
+struct Parts {
+    hipStream_t s[kMaxParts] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kMaxParts] = {};
+    int made = 0;
+};
+
+std::mutex g_parts_mu;
+std::map<std::pair<int, hipStream_t>, Parts> g_parts;
+
+// Streams and events of `stream` on the current device, at least `parts` of
+// them (g_parts_mu held).
+hipError_t parts_of(hipStream_t stream, int parts, Parts** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Parts& p = g_parts[{dev, stream}];
+    if (!p.fork) {
+        e = hipEventCreateWithFlags(&p.fork, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            p.fork = nullptr;
+            return e;
+        }
+    }
+    while (p.made < parts) {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(s);
+            return e;
+        }
+        p.s[p.made] = s;
+        p.join[p.made] = ev;
+        ++p.made;
+    }
+    *out = &p;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t fork_streams(hipStream_t stream, int parts, hipStream_t* subs) {
+    if (parts < 1 || parts > kMaxParts) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_parts_mu);
+    Parts* p = nullptr;
+    hipError_t e = parts_of(stream, parts, &p);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(p->fork, stream);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < parts; ++i) {
+        e = hipStreamWaitEvent(p->s[i], p->fork, 0);
+        if (e != hipSuccess) return e;
+        subs[i] = p->s[i];
+    }
+    return hipSuccess;
+}
+
+hipError_t join_streams(hipStream_t stream, int parts, const hipStream_t* subs) {
+    if (parts < 1 || parts > kMaxParts) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_parts_mu);
+    Parts* p = nullptr;
+    hipError_t e = parts_of(stream, parts, &p);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < parts; ++i) {
+        if (subs[i] != p->s[i]) return hipErrorInvalidValue;
+        e = hipEventRecord(p->join[i], p->s[i]);
+        if (e != hipSuccess) return e;
+        e = hipStreamWaitEvent(stream, p->join[i], 0);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int part_count(int batch, int chunk, int want) {
+    if (want < 1) return 1;
+    if (want > kMaxParts) want = kMaxParts;
+    if (chunk < batch) return 1;
+    while (want > 1 && batch / want < kMinPartPairs) --want;
+    return want;
+}
+
 hipError_t workspace(hipStream_t stream, size_t bytes, void** out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);

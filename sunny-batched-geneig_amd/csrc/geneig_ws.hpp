@@ -28,4 +28,22 @@ hipError_t workspace(hipStream_t stream, size_t bytes, void** out);
 // no limit below the compiled capacity.
 int ql_log_entries();
 
+// Concurrent parts of one launch.  A batch that fits one workspace chunk can
+// be cut into parts whose kernel pipelines run on separate streams, so one
+// part's memory-bound or low-occupancy stages overlap another's compute-bound
+// ones.  fork_streams hands out `parts` (<= kMaxParts) internal streams of the
+// caller's stream, kept per (device, caller stream) for the process lifetime
+// (non-blocking, default priority), each made to wait for the work already
+// queued on the caller's stream; join_streams makes the caller's stream wait
+// for everything queued on them.  The record/wait pairs go through events,
+// so the pattern is also valid under stream capture.
+constexpr int kMaxParts = 4;
+hipError_t fork_streams(hipStream_t stream, int parts, hipStream_t* subs);
+hipError_t join_streams(hipStream_t stream, int parts, const hipStream_t* subs);
+
+// Parts to cut a batch into: `want` when every part keeps at least
+// kMinPartPairs pairs and the batch is one workspace chunk, else 1.
+constexpr int kMinPartPairs = 8192;
+int part_count(int batch, int chunk, int want);
+
 }  // namespace geneig_host

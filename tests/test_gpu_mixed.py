@@ -195,3 +195,27 @@ def test_runtime_precision_vs_oracle(geneig, oracle, device, n, cond):
         assert mm[k] <= max(bar, 2.0 * mp[k]), k
     if cond <= 1e4:
         assert fb == 0
+
+
+def test_mixed_concurrent_parts_match_single_launches(geneig, oracle, device):
+    """The precision form of a one-chunk batch runs as three concurrent parts
+    (Cholesky, complex64 stages and refinement per part on internal streams,
+    fallback list shared); 30,000 pairs must equal three separate launches of
+    10,000 (each too small to split) bit for bit, step counts included."""
+    import torch
+
+    total, part = 30000, 10000
+    A, B = geneig.gen_spinwave(27, total, seed=10)
+    W, U, info = geneig.empty_outputs(54, total, device)
+    iters = torch.full((total,), -1, dtype=torch.int32, device=device)
+    fb = geneig.mixed_n54_batched(A, B, W, U, info, iters=iters)
+    W2, U2, i2 = geneig.empty_outputs(54, total, device)
+    it2 = torch.full((total,), -1, dtype=torch.int32, device=device)
+    fb2 = 0
+    for off in range(0, total, part):
+        sl = slice(off, off + part)
+        fb2 += geneig.mixed_n54_batched(A[sl], B[sl], W2[sl], U2[sl], i2[sl], iters=it2[sl])
+    torch.cuda.synchronize()
+    assert fb == fb2
+    assert torch.equal(info, i2) and torch.equal(iters, it2) and int(iters.min()) >= 1
+    assert torch.equal(W, W2) and torch.equal(U, U2)

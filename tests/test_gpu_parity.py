@@ -283,3 +283,32 @@ def test_ql_log_overflow_path(geneig, oracle, device, monkeypatch):
         res[cap] = (to_host(W), to_host(Wm), to_host(Wg))
     for a, b in zip(res[None], res["16"]):
         assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+
+
+def test_concurrent_parts_match_single_launches(geneig, oracle, device):
+    """A batch that fits one workspace chunk runs as concurrent parts on
+    internal streams forked from the caller's stream and joined back: 30,000
+    pairs go as three parts of 10,000, which must equal three separate launches
+    of 10,000 pairs (each too small to split) bit for bit, in outputs and info,
+    with the caller's stream ordering intact (results read on it right after)."""
+    import torch
+
+    n, total, part = 54, 30000, 10000
+    A, B = geneig.gen_spinwave(27, total, seed=9)
+    W, U, info = geneig.empty_outputs(n, total, device)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        geneig.full_n54_batched(A, B, W, U, info, stream=s)
+        Wc, Uc, ic = W.clone(), U.clone(), info.clone()      # queued on s after the join
+    s.synchronize()
+    W2, U2, i2 = geneig.empty_outputs(n, total, device)
+    for off in range(0, total, part):
+        sl = slice(off, off + part)
+        geneig.full_n54_batched(A[sl], B[sl], W2[sl], U2[sl], i2[sl])
+    torch.cuda.synchronize()
+    assert torch.equal(ic, i2) and int(ic.abs().max()) == 0
+    assert torch.equal(Wc, W2) and torch.equal(Uc, U2)
+    Ah, Bh = A[:64].cpu().numpy(), B[:64].cpu().numpy()
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    check_pairs(oracle, Ah, Bh, to_host(Wc[:64]), to_host(Uc[:64]), Wo, Xo, n, "parts")
