@@ -270,6 +270,7 @@ def bench_learned(args, geneig, torch, dev):
                    "network": "8 -> 256 -> 256 -> 256 (tanh) -> 5832, FP32", "train_epochs": args.train_epochs,
                    "train_batch": 256, "train_seconds": train_s, "parallelism": "none (1 GPU)"},
         "train_loss_first_last": [hist[0], hist[-1]],
+        "train_loss_best": m.train_best, "train_rollbacks": m.rollbacks,
         "jacobi_sweeps_mean": {"network": float(sw_l.mean()), "identity": float(sw_i.mean())},
         "jacobi_sweeps_saved_mean": float(sw_i.mean() - sw_l.mean()),
         "jacobi_sweeps_histogram": {"network": hist_of(sw_l), "identity": hist_of(sw_i)},

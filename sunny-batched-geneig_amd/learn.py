@@ -179,22 +179,33 @@ class WarmStart:
         self.adam(lr, stream=stream)
         return loss
 
-    def train(self, X, Cset, epochs, batch=256, lr=1e-3, lr_final=1e-4, seed=0, log=None):
+    def train(self, X, Cset, epochs, batch=256, lr=1e-3, lr_final=1e-4, seed=0, log=None, spike=1.5):
         """Adam over shuffled mini-batches, learning rate decayed geometrically
-        from lr to lr_final.  Returns the mean loss per epoch."""
+        from lr to lr_final.  Returns the mean loss per epoch.
+
+        Loss spikes: a network output close to rank deficiency makes the
+        Cholesky-QR gradient huge, and one such step can throw the weights far
+        off (the loss then stays high).  The weights and Adam moments of the
+        best epoch so far are kept; an epoch whose mean loss exceeds `spike`
+        times the best returns to them with the learning rate halved from then
+        on, and training ends on the best epoch's weights (``train_best`` is
+        its mean loss; ``rollbacks`` counts the returns)."""
         torch = _torch()
         P = X.shape[0]
         gen = torch.Generator(device="cpu").manual_seed(seed)
         steps = epochs * ((P + batch - 1) // batch)
         k = 0
         hist = []
+        best = None
+        factor = 1.0
+        self.rollbacks = 0
         for ep in range(epochs):
             perm = torch.randperm(P, generator=gen).to(torch.int32).to(self.device)
             tot = torch.zeros((), dtype=torch.float64, device=self.device)
             cnt = 0
             for off in range(0, P, batch):
                 idx = perm[off:off + batch].contiguous()
-                rate = lr * (lr_final / lr) ** (k / max(1, steps - 1))
+                rate = factor * lr * (lr_final / lr) ** (k / max(1, steps - 1))
                 l = self.train_step(X.index_select(0, idx.long()).contiguous(), Cset, idx, rate)
                 tot += torch.nan_to_num(l.double(), nan=0.0).sum()
                 cnt += idx.numel()
@@ -202,7 +213,26 @@ class WarmStart:
             hist.append(float((tot / cnt).item()))
             if log:
                 log("epoch %d: mean loss %.5f" % (ep, hist[-1]))
+            if best is None or hist[-1] < best[0]:
+                best = (hist[-1], self.flat.clone(), self.m.clone(), self.v.clone(), self.step)
+            elif hist[-1] > spike * best[0]:
+                self._restore(best)
+                factor *= 0.5
+                self.rollbacks += 1
+                if log:
+                    log("epoch %d: loss spike (best %.5f): back to the best weights, learning rate x%.3g"
+                        % (ep, best[0], factor))
+        if best is not None and hist[-1] > best[0]:
+            self._restore(best)
+        self.train_best = best[0] if best is not None else float("nan")
         return hist
+
+    def _restore(self, state):
+        _, flat, m, v, step = state
+        self.flat.copy_(flat)
+        self.m.copy_(m)
+        self.v.copy_(v)
+        self.step = step
 
 
 def learned_n54_batched(H, S, W, U, info, X=None, model=None, max_sweeps=12, jacobi_tol=1e-6, max_iter=0, tol=0.0,
