@@ -520,8 +520,8 @@ def main():
             traffic = None
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK, "traffic": traffic,
-                "kernel": "one chosen-form solve launch (n=54): chol64 + k1f/k2af/k2bf/k2xf/k3f per chunk + "
-                          "refine (+ FP64 k1l/k2al/k2bl/k2xl/k3l over fallback pairs)",
+                "kernel": "one chosen-form solve launch (n=54): chol64 -> k1f/k2af/k2bf/k2xf/k3f -> refine on each "
+                          "of three concurrent parts (+ FP64 k1l/k2al/k2bl/k2xl/k3l over fallback pairs)",
                 "flop_per_pair": flops_per_pair(n), "pairs_per_launch": batch,
                 "avg_launch_ms": launch_c * 1e3,
                 "hbm_achieved_GBps": bytes_per_pair(n) * batch / launch_c / 1e9,
