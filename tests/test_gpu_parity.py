@@ -312,3 +312,36 @@ def test_concurrent_parts_match_single_launches(geneig, oracle, device):
     Ah, Bh = A[:64].cpu().numpy(), B[:64].cpu().numpy()
     Wo, Xo, _ = oracle.zhegv(Ah, Bh)
     check_pairs(oracle, Ah, Bh, to_host(Wc[:64]), to_host(Uc[:64]), Wo, Xo, n, "parts")
+
+
+def test_concurrent_parts_under_graph_capture(geneig, oracle, device):
+    """The fork and join of the concurrent parts are event record/wait pairs, so
+    a batched launch captured on a stream becomes a graph with the parts as
+    parallel branches.  After one eager launch on that stream (its internal
+    streams and workspace then exist), a 16,384-pair launch (two parts) is
+    captured and replayed three times, with eager work and synchronizations
+    between the replays; every replay equals the eager result bit for bit."""
+    import torch
+
+    n, total = 54, 16384
+    A, B = geneig.gen_spinwave(27, total, seed=12)
+    W0, U0, i0 = geneig.empty_outputs(n, total, device)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        geneig.full_n54_batched(A, B, W0, U0, i0, stream=s)
+    s.synchronize()
+    W, U, info = geneig.empty_outputs(n, total, device)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        geneig.full_n54_batched(A, B, W, U, info, stream=s)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        W.fill_(float("nan"))
+        U.fill_(float("nan"))
+        info.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(W, W0) and torch.equal(U, U0) and torch.equal(info, i0), rep
+    assert int(i0.abs().max()) == 0
