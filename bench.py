@@ -417,7 +417,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: pairs are independent, a multi-GPU run would "
-              "split the batch into per-GPU ranges with no exchange (see DESIGN.md); "
+              "split the batch into per-GPU ranges with no exchange; "
               "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

@@ -3,8 +3,8 @@
  * generalized eigensolve  A x = lambda B x  (B Hermitian positive definite).
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this code.  It is used by tests/ (as the checker), by
- * __graft_entry__.smoke() (as the checker) and by bench.py's cpu_baseline leg.
+ * calls this code.  It is used by tests/ (to check results), by
+ * __graft_entry__.smoke() (to check its result) and by bench.py's cpu_baseline leg.
  *
  * What it restates: the reference pipeline of
  *   src/nvrtc/kernel_source.hpp:240-248 (and src/geneig_fixed.cu:455-464):

@@ -13,8 +13,8 @@ kernels of sunny-batched-geneig_amd/csrc/geneig_learn.hip compute:
                                 dloss/dIm Z = Im G
   jacobi(C, Q0, tol, max_sweeps) round-robin cyclic Jacobi on Q0^H C Q0
 
-The reference has no counterpart (the learned warm start is a north-star
-addition); the loss gradient and the network backward are pinned by finite
+The reference has no counterpart (the learned warm start is an addition of
+this library); the loss gradient and the network backward are pinned by finite
 differences (tests/test_learn_oracle.py), the Jacobi by numpy.linalg.eigh.
 """
 import numpy as np

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() (as
-the checker) and by bench.py's cpu_baseline leg.  The product library never
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() (to
+check its result) and by bench.py's cpu_baseline leg.  The product library never
 imports it.  Arrays are numpy, column-major per pair: a batch of n x n
 matrices is an array of shape (batch, n, n) whose pair b, element (i, j) is
 ``M[b, j, i]`` (i.e. each pair stored as its transpose in C order), exactly

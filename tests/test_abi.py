@@ -105,7 +105,7 @@ def test_solver_create_reports_errors(geneig):
 
 
 def test_product_does_not_touch_oracle():
-    """The product sources never reference the oracle (it is the checker only)."""
+    """The product sources never reference the oracle (it only checks results)."""
     pkg = os.path.join(ROOT, "sunny-batched-geneig_amd")
     for dirpath, _, files in os.walk(pkg):
         for f in files:
