@@ -15,8 +15,17 @@
  *                 (LAPACK zhegv numbering).  Written for every pair, so the
  *                 caller no longer has to zero it.
  * Every entry point that launches work returns a hipError_t value (0 = success).
- * batch <= 0 is a no-op returning 0.  Device pointers only; the library
- * allocates nothing per launch.
+ * batch <= 0 is a no-op returning 0.  Device pointers only.
+ *
+ * Workspace: the solve stages hand their intermediates (tridiagonal, Q,
+ * rotation log, Z) over in a device workspace the library keeps per
+ * (device, stream): about 120 KB per pair at n = 54 (FP64), bounded by a
+ * 16 GiB budget (GENEIG_WORKSPACE_MB overrides; larger batches run in
+ * chunks).  It is allocated by the first launch on a stream that needs it and
+ * grown by a later, larger one; geneig_n54_workspace_bytes gives the size,
+ * geneig_n54_reserve allocates it ahead of time (a launch after it allocates
+ * nothing and can be captured into a graph), geneig_workspace_release frees
+ * it.  The reference's kernel used shared memory only and had no workspace.
  */
 #ifndef GENEIG_H
 #define GENEIG_H
@@ -84,6 +93,28 @@ int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex
                              int batch_size, hipStream_t stream);
 unsigned int geneig_finish_n54_block_dim_x(void);       /* geneig_fixed.cu:590 */
 unsigned int geneig_finish_n54_shared_mem_bytes(void);  /* geneig_fixed.cu:591 */
+
+/* ======================================================================== */
+/* Workspace control.  No counterpart in the reference (whose kernels used  */
+/* shared memory only); callers that capture launches into graphs or must   */
+/* bound device memory use these.                                           */
+/* ======================================================================== */
+
+/* Workspace bytes a batched n = 54 launch of batch_size pairs uses on its
+ * stream: form 0 geneig_full_n54_batched_launch, form 1
+ * geneig_mixed_n54_batched_launch (geneig_mixed.h). */
+size_t geneig_n54_workspace_bytes(int batch_size, int form);
+/* Allocates that workspace on `stream` (and the internal streams of the
+ * concurrent parts) ahead of time.  A launch of at most batch_size pairs on
+ * `stream` afterwards allocates nothing; growing the workspace while the
+ * stream is being captured is refused (hipErrorStreamCaptureUnsupported). */
+int geneig_n54_reserve(int batch_size, int form, hipStream_t stream);
+/* Frees the workspace held for `stream` on the current device, after waiting
+ * for the stream (also the buffers a captured stream outgrew, which the
+ * library keeps while a graph may use them).  *freed_bytes (nullable)
+ * receives the bytes freed.  The caller must not replay graphs captured on
+ * `stream` afterwards.  Works for the workspaces of runtime solvers too. */
+int geneig_workspace_release(hipStream_t stream, size_t* freed_bytes);
 
 #ifdef __cplusplus
 }

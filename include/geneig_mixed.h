@@ -15,7 +15,11 @@
  *   4. pairs whose refinement diverges, stalls after max_iter steps, or meets
  *      coinciding eigenvalues are solved again by the FP64 pipeline.
  *
- * The launch synchronizes `stream` once, after step 3, to size step 4.
+ * Step 4 reads the number of listed pairs on the device (its kernels walk
+ * the list with fixed grids), so the launch does not wait for the device
+ * unless h_fallback is given: then it waits once, at the end, to report the
+ * count.  With h_fallback NULL the launch is asynchronous and can be captured
+ * into a graph (after geneig_n54_reserve on the capturing stream).
  * max_iter <= 0 selects 4, tol <= 0 selects 1e-7 (the refined X then carries
  * errors of order tol^2).  d_iters (nullable, device, batch_size ints)
  * receives the refinement steps taken per pair (0: handed to step 4 before

@@ -65,8 +65,7 @@ int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void*
  * device staging slot of chunk_size pairs per stream (allocated on first use,
  * grown on demand, freed by geneig_solver_destroy).  Returns once everything
  * is enqueued; the host outputs are complete after every stream is
- * synchronized.  On a precision-form solver each chunk's launch waits for its
- * own fallback count, which serializes the chunks. */
+ * synchronized. */
 int geneig_solver_launch_chunked_host(geneig_solver_t s, const void* h_H, const void* h_S,
                                       double* h_W, void* h_U, int* h_info, int total_matrices,
                                       int chunk_size, int num_streams, const hipStream_t* streams);
@@ -76,6 +75,10 @@ int geneig_solver_launch_chunked_host(geneig_solver_t s, const void* h_H, const 
  * and can be captured into a graph.  No counterpart in the reference (whose
  * kernel used shared memory only). */
 int geneig_solver_reserve(geneig_solver_t s, int batch_size, hipStream_t stream);
+/* Workspace bytes a launch of batch_size pairs uses on its stream (what
+ * geneig_solver_reserve allocates); geneig_workspace_release (geneig.h)
+ * frees a stream's workspace. */
+size_t geneig_solver_workspace_bytes(geneig_solver_t s, int batch_size);
 
 /* Accessors (nvrtc_solver.hpp:120-140). */
 int geneig_solver_matrix_size(geneig_solver_t s);

@@ -87,6 +87,10 @@ def lib():
         "geneig_solver_acquire_module_ms": (D, [P]),
         "geneig_solver_variant": (I, [P]),
         "geneig_solver_reserve": (I, [P, I, P]),
+        "geneig_solver_workspace_bytes": (C.c_size_t, [P, I]),
+        "geneig_n54_workspace_bytes": (C.c_size_t, [I, I]),
+        "geneig_n54_reserve": (I, [I, I, P]),
+        "geneig_workspace_release": (I, [P, P]),
         "geneig_multi_create": (I, [P, I, P, I, I, I]),
         "geneig_multi_destroy": (None, [P]),
         "geneig_multi_launch": (I, [P, P, P, P, P, P, P, P]),
@@ -209,12 +213,16 @@ def full_n54_batched(H, S, W, U, info, batch=None, stream=None):
                "geneig_full_n54_batched_launch")
 
 
-def mixed_n54_batched(H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None):
+def mixed_n54_batched(H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None,
+                      report_fallback=True):
     """Precision form of the n = 54 batched solve (geneig_mixed_n54_batched_launch):
     complex64 bulk, FP64 Cholesky and FP64 refinement, FP64 re-solve of the
     pairs the refinement hands back.  Same outputs as ``full_n54_batched``.
     ``iters`` (optional int32 device tensor of ``batch`` entries) receives the
-    refinement steps per pair.  Returns the number of pairs solved again in FP64."""
+    refinement steps per pair.  Returns the number of pairs solved again in
+    FP64, for which the launch waits on the device; with
+    ``report_fallback=False`` it returns None and does not wait (the launch
+    is then asynchronous and capturable)."""
     batch = H.shape[0] if batch is None else batch
     _check_batch(54, batch, H, S, W, U, info)
     if iters is not None:
@@ -225,9 +233,27 @@ def mixed_n54_batched(H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=N
     _check_hip(lib().geneig_mixed_n54_batched_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
                                                      int(max_iter), float(tol),
                                                      _ptr(iters) if iters is not None else None,
-                                                     C.byref(fb), _stream(stream)),
+                                                     C.byref(fb) if report_fallback else None, _stream(stream)),
                "geneig_mixed_n54_batched_launch")
-    return fb.value
+    return fb.value if report_fallback else None
+
+
+def n54_workspace_bytes(batch, form=0):
+    """Workspace bytes of a batched n = 54 launch (form 0 plain FP64, 1 precision)."""
+    return int(lib().geneig_n54_workspace_bytes(int(batch), int(form)))
+
+
+def n54_reserve(batch, form=0, stream=None):
+    """Allocate the workspace of a batched n = 54 launch of ``batch`` pairs on ``stream``
+    ahead of time (the launch then allocates nothing and can be captured)."""
+    _check_hip(lib().geneig_n54_reserve(int(batch), int(form), _stream(stream)), "geneig_n54_reserve")
+
+
+def workspace_release(stream=None):
+    """Free the workspace the library holds for ``stream`` on the current device; returns bytes freed."""
+    freed = C.c_size_t(0)
+    _check_hip(lib().geneig_workspace_release(_stream(stream), C.byref(freed)), "geneig_workspace_release")
+    return freed.value
 
 
 def full_n54(H, S, W, U, info, stream=None):
@@ -307,6 +333,9 @@ class GeneigSolver:
 
     def matrix_size(self):
         return lib().geneig_solver_matrix_size(self._h)
+
+    def workspace_bytes(self, batch):
+        return int(lib().geneig_solver_workspace_bytes(self._h, int(batch)))
 
     def batches_per_block(self):
         return lib().geneig_solver_batches_per_block(self._h)

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <set>
 
@@ -40,7 +41,7 @@ __global__ void __launch_bounds__(64 * kBPB) k1_kernel(const geneig::cplx* __res
                                                         int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ geneig::cplx vecs[kBPB * C54::VEC];
-    geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, smem, vecs);
+    geneig::k1_batched<kN, kBPB, MODE>(A, B, W, U, info, ws, batch, batch, smem, vecs);
 }
 
 // The FP64 Cholesky alone (two waves per SIMD); the batched FP64 solve runs it
@@ -51,7 +52,7 @@ __global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(
                                                                            int* __restrict__ info, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ geneig::cplx vecs[C54::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, smem, vecs);
+    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, batch, smem, vecs);
 }
 
 // K2 in three kernels: the QL chases of 64 pairs per wave with the rotations
@@ -64,21 +65,26 @@ constexpr unsigned kK1Lds = geneig::k1_lds_bytes<kN, double>();
 constexpr unsigned kWorkLds = kK1Lds > kK3Lds ? kK1Lds : kK3Lds;
 constexpr unsigned kVecLds = sizeof(geneig::cplx) * C54::VEC;
 
-__global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch, int cap) {
+__global__ void __launch_bounds__(64) k2a_kernel(int* __restrict__ info, unsigned char* ws, int batch, int cap,
+                                                  int* __restrict__ ovf) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2a_batched<kN, double>(info, ws, batch, reinterpret_cast<double*>(smem), cap);
+    geneig::k2a_batched<kN, double>(info, ws, batch, batch, reinterpret_cast<double*>(smem), cap, ovf);
 }
 
 __global__ void __launch_bounds__(64 * C54::R, GENEIG_K2B_WAVES_PER_EU) k2b_kernel(const int* __restrict__ info,
                                                                                  unsigned char* ws, int batch) {
     __shared__ geneig::cplx buf[geneig::k2b_buf_entries<kN, double>()];
-    geneig::k2b_batched<kN, double>(info, ws, batch, buf);
+    geneig::k2b_batched<kN, double>(info, ws, batch, batch, buf);
 }
 
-__global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2x_kernel(int* __restrict__ info, unsigned char* ws, int batch) {
+__global__ void __launch_bounds__(64 * C54::R, GENEIG_K2_WAVES_PER_EU) k2x_kernel(int* __restrict__ info, unsigned char* ws,
+                                                                                int chunk, const int* __restrict__ ovf) {
     __shared__ geneig::cplx rotlog[C54::R * C54::K2_VEC];
-    geneig::k2x_batched<kN>(info, ws, batch, rotlog);
+    geneig::k2x_batched<kN>(info, ws, chunk, rotlog, ovf);
 }
+
+// Clears n ints with kernel stores (a captured launch then holds no memset node).
+__global__ void zero_kernel(int* p, int n) { geneig::zero_ints(p, n); }
 
 template <int MODE>
 __global__ void __launch_bounds__(64 * kBPB, GENEIG_K3_WAVES_PER_EU) k3_kernel(const geneig::cplx* __restrict__ B,
@@ -87,7 +93,7 @@ __global__ void __launch_bounds__(64 * kBPB, GENEIG_K3_WAVES_PER_EU) k3_kernel(c
                                                         int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ geneig::cplx vecs[kBPB * C54::VEC];
-    geneig::k3_batched<kN, kBPB, MODE>(B, W, U, info, ws, batch, smem, vecs);
+    geneig::k3_batched<kN, kBPB, MODE>(B, W, U, info, ws, batch, batch, smem, vecs);
 }
 
 // hipFuncSetAttribute once per kernel per process, thread-safe (the reference
@@ -109,79 +115,122 @@ hipError_t set_lds_once(const void* kernel, unsigned bytes) {
         if (_e != hipSuccess) return _e;     \
     } while (0)
 
-// The solve pipeline over a batch, chunked to bound the workspace.
-// MODE_FINISH takes M in A and L in B.
+// Whether a mode runs the eigen stages (K2, K3) after K1.
 template <int MODE>
-hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int batch, hipStream_t stream) {
-    // the batched FP64 solve: the Cholesky as its own kernel (two waves per
-    // SIMD), then K1 from the stashed factor; results are bit-identical (the
-    // same operations per entry, only the update chunking differs)
+constexpr bool eig_mode() {
+    return MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG || MODE == geneig::MODE_FINISH;
+}
+
+// Workspace layout of a launch of `batch` pairs (the batched FP64 solve runs
+// kPlainParts concurrent parts when the batch is one chunk).
+template <int MODE>
+geneig_host::PlainLayout layout_of(int batch) {
+    return geneig_host::plain_layout(batch, W54::PAIR_BYTES, MODE == geneig::MODE_FULL ? kPlainParts : 1);
+}
+
+template <int MODE>
+hipError_t set_attrs() {
     constexpr bool split_chol = (MODE == geneig::MODE_FULL);
     auto k1 = k1_kernel<split_chol ? geneig::MODE_MIXED : MODE>;
-    auto k3 = k3_kernel<MODE>;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), kK1Lds * kBPB));
     if (split_chol)
         RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3_kernel<MODE>), kK3Lds * kBPB));
+    return set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds);
+}
 
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3), kK3Lds * kBPB));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
-    if (batch <= 0) return hipSuccess;
-    constexpr bool eig = (MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG ||
-                          MODE == geneig::MODE_FINISH);
-    const int chunk = geneig_host::chunk_pairs(batch, W54::PAIR_BYTES);
-    // a batched solve that fits one chunk runs as kPlainParts concurrent
-    // parts (each with its own region of the workspace and its own stream)
-    const int parts = split_chol ? geneig_host::part_count(batch, chunk, kPlainParts) : 1;
-    const size_t region = [&](int P) { return ((size_t)P * W54::PAIR_BYTES + 255) / 256 * 256; }((batch + parts - 1) / parts);
-    void* ws = nullptr;
-    // every mode past the Cholesky uses the workspace (K1 stages W = L^{-1} A there)
-    if (MODE != geneig::MODE_CHOL)
-        RETURN_IF(geneig_host::workspace(stream, parts > 1 ? region * parts : (size_t)chunk * W54::PAIR_BYTES, &ws));
+// The stages for pairs [off, off + P) with solver region w (stage arrays for P
+// slots, then the control block holding K2a's overflow list) on stream st.
+template <int MODE>
+hipError_t pipeline(const geneig::cplx* a, const geneig::cplx* b, double* W, geneig::cplx* u, int* info, int off,
+                    int P, unsigned char* w, hipStream_t st) {
+    constexpr bool split_chol = (MODE == geneig::MODE_FULL);
+    auto k1 = k1_kernel<split_chol ? geneig::MODE_MIXED : MODE>;
     const size_t mat = (size_t)kN * kN;
+    const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
+    double* w_off = W ? W + (size_t)kN * off : nullptr;
+    int* ovf = geneig_host::region_ctl(w, P, W54::PAIR_BYTES);
+    if (eig_mode<MODE>()) {
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ovf, 1);
+        RETURN_IF(hipGetLastError());
+    }
+    if (split_chol) {
+        hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), st,
+                           b + mat * off, w_off, u + mat * off, info + off, P);
+        RETURN_IF(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, st, a + mat * off, b + mat * off, w_off,
+                       u + mat * off, info + off, w, P);
+    RETURN_IF(hipGetLastError());
+    if (!eig_mode<MODE>()) return hipSuccess;
+    hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, st, info + off, w, P,
+                       geneig_host::ql_log_entries(), ovf);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)std::min(P, geneig::kListGridK2x)), dim3(64 * C54::R), 0, st,
+                       info + off, w, P, (const int*)ovf);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k3_kernel<MODE>, dim3(g13), dim3(64, kBPB), kK3Lds * kBPB, st, b + mat * off, w_off,
+                       u + mat * off, info + off, w, P);
+    return hipGetLastError();
+}
+
+// The solve pipeline over a batch, chunked to bound the workspace.
+// MODE_FINISH takes M in A and L in B.  fixed_ws: a caller-provided
+// workspace of fixed_bytes (the allocation-free entry point); the launch then
+// runs on the caller's stream alone and allocates nothing.
+template <int MODE>
+hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int batch, hipStream_t stream,
+               unsigned char* fixed_ws = nullptr, size_t fixed_bytes = 0) {
+    RETURN_IF(set_attrs<MODE>());
+    if (batch <= 0) return hipSuccess;
+    geneig_host::PlainLayout L = fixed_ws ? geneig_host::plain_layout(batch, W54::PAIR_BYTES, 1) : layout_of<MODE>(batch);
+    void* ws = nullptr;
+    if (fixed_ws) {
+        if (L.chunk < batch || L.bytes > fixed_bytes) return hipErrorInvalidValue;
+        ws = fixed_ws;
+    } else if (MODE != geneig::MODE_CHOL) {   // every mode past the Cholesky uses the workspace
+        RETURN_IF(geneig_host::workspace(stream, L.bytes, &ws));
+    }
     const auto* a = static_cast<const geneig::cplx*>(A);
     const auto* b = static_cast<const geneig::cplx*>(B);
     auto* u = static_cast<geneig::cplx*>(U);
-    // the stages for pairs [off, off + P) with workspace w on stream st
-    auto pipeline = [&](int off, int P, unsigned char* w, hipStream_t st) -> hipError_t {
-        const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
-        if (split_chol) {
-            hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), st,
-                               b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, P);
-            RETURN_IF(hipGetLastError());
-        }
-        hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, st, a + mat * off,
-                           b + mat * off, W ? W + (size_t)kN * off : nullptr, u + mat * off, info + off, w, P);
-        RETURN_IF(hipGetLastError());
-        if (!eig) return hipSuccess;
-        hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, st, info + off, w, P,
-                           geneig_host::ql_log_entries());
-        RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
-        RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
-        RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3, dim3(g13), dim3(64, kBPB), kK3Lds * kBPB, st, b + mat * off,
-                           W + (size_t)kN * off, u + mat * off, info + off, w, P);
-        return hipGetLastError();
-    };
-    if (parts > 1) {
+    unsigned char* base = static_cast<unsigned char*>(ws);
+    if (L.parts > 1) {
         hipStream_t subs[geneig_host::kMaxParts];
-        RETURN_IF(geneig_host::fork_streams(stream, parts, subs));
+        RETURN_IF(geneig_host::fork_streams(stream, L.parts, subs));
         hipError_t e = hipSuccess;
-        for (int i = 0; i < parts && e == hipSuccess; ++i) {
-            const int off = (int)((long long)batch * i / parts);
-            const int P = (int)((long long)batch * (i + 1) / parts) - off;
-            e = pipeline(off, P, static_cast<unsigned char*>(ws) + region * i, subs[i]);
+        for (int i = 0; i < L.parts && e == hipSuccess; ++i) {
+            const int off = (int)((long long)batch * i / L.parts);
+            const int P = (int)((long long)batch * (i + 1) / L.parts) - off;
+            e = pipeline<MODE>(a, b, W, u, info, off, P, base + L.region * i, subs[i]);
         }
-        const hipError_t j = geneig_host::join_streams(stream, parts, subs);
+        const hipError_t j = geneig_host::join_streams(stream, L.parts, subs);
         return e != hipSuccess ? e : j;
     }
-    for (int off = 0; off < batch; off += chunk)
-        RETURN_IF(pipeline(off, std::min(chunk, batch - off), static_cast<unsigned char*>(ws), stream));
+    for (int off = 0; off < batch; off += L.chunk)
+        RETURN_IF(pipeline<MODE>(a, b, W, u, info, off, std::min(L.chunk, batch - off), base, stream));
     return hipSuccess;
 }
 
 }  // namespace
+
+namespace geneig_host {
+
+// The batched FP64 n = 54 solve into a caller-provided workspace of `bytes`
+// (one region, the caller's stream only, nothing allocated).  Returns
+// hipErrorInvalidValue when the batch's workspace does not fit.
+hipError_t full_n54_into(const void* H, const void* S, double* W, void* U, int* info, int batch,
+                         hipStream_t stream, unsigned char* ws, size_t bytes) {
+    if (!ws) return hipErrorInvalidValue;
+    return run<geneig::MODE_FULL>(H, S, W, U, info, batch, stream, ws, bytes);
+}
+
+// Workspace bytes of full_n54_into for `batch` pairs.
+size_t full_n54_into_bytes(int batch) { return plain_layout(batch, W54::PAIR_BYTES, 1).bytes; }
+
+}  // namespace geneig_host
 
 extern "C" {
 
@@ -234,5 +283,25 @@ int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex
 }
 unsigned int geneig_finish_n54_block_dim_x(void) { return 64; }
 unsigned int geneig_finish_n54_shared_mem_bytes(void) { return (kWorkLds + kVecLds) * kBPB; }
+
+size_t geneig_n54_workspace_bytes(int batch_size, int form) {
+    if (batch_size <= 0) return 0;
+    if (form == 1) return geneig_host::mixed_n54_layout(batch_size).bytes;
+    return layout_of<geneig::MODE_FULL>(batch_size).bytes;
+}
+
+int geneig_n54_reserve(int batch_size, int form, hipStream_t stream) {
+    if (form != 0 && form != 1) return hipErrorInvalidValue;
+    if (batch_size <= 0) return hipSuccess;
+    const int parts = form == 1 ? geneig_host::mixed_n54_layout(batch_size).parts
+                                : layout_of<geneig::MODE_FULL>(batch_size).parts;
+    RETURN_IF(geneig_host::prepare_parts(stream, parts));
+    void* ws = nullptr;
+    return geneig_host::workspace(stream, geneig_n54_workspace_bytes(batch_size, form), &ws);
+}
+
+int geneig_workspace_release(hipStream_t stream, size_t* freed_bytes) {
+    return geneig_host::release(stream, freed_bytes);
+}
 
 }  // extern "C"

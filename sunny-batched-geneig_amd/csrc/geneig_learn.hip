@@ -463,21 +463,21 @@ __global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(
                                                      cplx* __restrict__ U, int* __restrict__ info, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, smem, vecs);
+    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, batch, smem, vecs);
 }
 __global__ void __launch_bounds__(64) k1c_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                   double* __restrict__ W, cplx* __restrict__ U,
                                                   int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplxf vecs[CF::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_MIXED_C, float>(A, B, W, U, info, ws, batch, smem, vecs);
+    geneig::k1_batched<kN, 1, geneig::MODE_MIXED_C, float>(A, B, W, U, info, ws, batch, batch, smem, vecs);
 }
 __global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3j_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
                                                   unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplxf vecs[CF::VEC];
-    geneig::k3_batched<kN, 1, geneig::MODE_JACOBI, float>(B, W, U, info, ws, batch, smem, vecs);
+    geneig::k3_batched<kN, 1, geneig::MODE_JACOBI, float>(B, W, U, info, ws, batch, batch, smem, vecs);
 }
 __global__ void __launch_bounds__(64 * RC::NT, geneig::refine_waves_per_eu<kN>()) refine_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                                  double* __restrict__ W, cplx* __restrict__ U,
@@ -489,26 +489,36 @@ __global__ void __launch_bounds__(64 * RC::NT, geneig::refine_waves_per_eu<kN>()
 }
 __global__ void __launch_bounds__(64) k1l_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                   double* __restrict__ W, cplx* __restrict__ U,
-                                                  int* __restrict__ info, unsigned char* ws, int count,
-                                                  const int* __restrict__ list) {
+                                                  int* __restrict__ info, unsigned char* ws, int chunk,
+                                                  const int* __restrict__ list, const int* __restrict__ dcount,
+                                                  int off) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_FULL, double>(A, B, W, U, info, ws, count, smem, vecs, list);
+    geneig::k1_batched<kN, 1, geneig::MODE_FULL, double, true>(
+        A, B, W, U, info, ws,
+        geneig::dispatch_slots(0, dcount, off, chunk), chunk, smem,
+                                                         vecs, list + off);
 }
 __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2l_kernel(int* __restrict__ info,
-                                                                               unsigned char* ws, int count,
-                                                                               const int* __restrict__ list) {
+                                                                               unsigned char* ws, int chunk,
+                                                                               const int* __restrict__ list,
+                                                                               const int* __restrict__ dcount,
+                                                                               int off) {
     __shared__ cplx rotlog[CD::R * CD::K2_VEC];
-    geneig::k2_batched<kN, double>(info, ws, count, rotlog, list);
+    geneig::k2_batched<kN, double, true>(info, ws, geneig::dispatch_slots(0, dcount, off, chunk), chunk, rotlog,
+                                   list + off);
 }
 __global__ void __launch_bounds__(64, GENEIG_K3_WAVES_PER_EU) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
-                                                  unsigned char* ws, int count, const int* __restrict__ list) {
+                                                  unsigned char* ws, int chunk, const int* __restrict__ list,
+                                                  const int* __restrict__ dcount, int off) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k3_batched<kN, 1, geneig::MODE_FULL, double>(B, W, U, info, ws, count, smem, vecs, list);
+    geneig::k3_batched<kN, 1, geneig::MODE_FULL, double, true>(B, W, U, info, ws,
+                                                         geneig::dispatch_slots(0, dcount, off, chunk), chunk, smem,
+                                                         vecs, list + off);
 }
-__global__ void zero_kernel(int* p) { *p = 0; }
+__global__ void zero_kernel(int* p, int n) { geneig::zero_ints(p, n); }
 
 std::mutex g_attr_mu;
 std::set<const void*> g_attr_done;
@@ -658,21 +668,19 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
     const int F = kF, Hd = kHidden;
     // per pair in a chunk: the FP32 workspace, plus network activations and output
     const unsigned long long mlp_pair = d_features ? sizeof(float) * (3ull * Hd + kOut) : 0ull;
-    const size_t list_bytes = ((sizeof(int) * ((size_t)batch + 1) + 255) / 256) * 256;
-    const int chunk_f = geneig_host::chunk_pairs(batch, WF::PAIR_BYTES + mlp_pair);
-    const int chunk_d = geneig_host::chunk_pairs(batch, WD::PAIR_BYTES);
-    const size_t solver_bytes = std::max((size_t)chunk_f * (WF::PAIR_BYTES + mlp_pair), (size_t)chunk_d * WD::PAIR_BYTES);
+    const geneig_host::MixedLayout L = geneig_host::mixed_layout(batch, WF::PAIR_BYTES, WD::PAIR_BYTES, 1, mlp_pair);
+    const int chunk_f = L.chunk_f;
     void* ws = nullptr;
-    RETURN_IF(geneig_host::workspace(stream, list_bytes + solver_bytes, &ws));
+    RETURN_IF(geneig_host::workspace(stream, L.bytes, &ws));
     int* fb_count = static_cast<int*>(ws);
     int* fb_list = fb_count + 1;
-    unsigned char* solver = static_cast<unsigned char*>(ws) + list_bytes;
+    unsigned char* solver = static_cast<unsigned char*>(ws) + L.list_bytes;
     const auto* A = reinterpret_cast<const cplx*>(d_H);
     const auto* B = reinterpret_cast<const cplx*>(d_S);
     auto* U = reinterpret_cast<cplx*>(d_U);
     const size_t mat = (size_t)kN * kN;
 
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, stream, fb_count);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, fb_count, 1);
     RETURN_IF(hipGetLastError());
     hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B, d_W, U, d_info, batch);
     RETURN_IF(hipGetLastError());
@@ -695,22 +703,28 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
     hipLaunchKernelGGL(refine_kernel, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, d_W, U,
                        (const int*)d_info, fb_list, fb_count, d_iters, batch, max_iter, tol);
     RETURN_IF(hipGetLastError());
-    int count = 0;
-    RETURN_IF(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
-    RETURN_IF(hipStreamSynchronize(stream));
-    if (count < 0 || count > batch) return hipErrorUnknown;
-    if (h_fallback) *h_fallback = count;
-    for (int off = 0; off < count; off += chunk_d) {
-        const int P = std::min(chunk_d, count - off);
-        hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), kK1LdsD, stream, A, B, d_W, U, d_info, solver, P,
-                           (const int*)(fb_list + off));
+    // the FP64 re-solve of the listed pairs, its length read on the device
+    const int cd = L.chunk_d;
+    const unsigned g1 = (unsigned)std::min(cd, geneig::kListGridK1);
+    const unsigned g2 = (unsigned)std::min(cd, geneig::kListGridK2x);
+    for (int r = 0; r < L.rounds; ++r) {
+        const int off = r * cd;
+        hipLaunchKernelGGL(k1l_kernel, dim3(g1), dim3(64), kK1LdsD, stream, A, B, d_W, U, d_info, solver, cd,
+                           (const int*)fb_list, (const int*)fb_count, off);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2l_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, d_info, solver, P,
-                           (const int*)(fb_list + off));
+        hipLaunchKernelGGL(k2l_kernel, dim3(g2), dim3(64 * CD::R), 0, stream, d_info, solver, cd,
+                           (const int*)fb_list, (const int*)fb_count, off);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), kK3LdsD, stream, B, d_W, U, (const int*)d_info,
-                           solver, P, (const int*)(fb_list + off));
+        hipLaunchKernelGGL(k3l_kernel, dim3(g1), dim3(64), kK3LdsD, stream, B, d_W, U, (const int*)d_info, solver, cd,
+                           (const int*)fb_list, (const int*)fb_count, off);
         RETURN_IF(hipGetLastError());
+    }
+    if (h_fallback) {
+        int count = 0;
+        RETURN_IF(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+        RETURN_IF(hipStreamSynchronize(stream));
+        if (count < 0 || count > batch) return hipErrorUnknown;
+        *h_fallback = count;
     }
     return hipSuccess;
 }

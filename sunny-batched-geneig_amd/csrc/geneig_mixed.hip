@@ -9,7 +9,8 @@
 //   K3f  sort, Y = Q Z, X0 = L^{-H} Y (complex64 arithmetic, complex128 out)
 //   RF   FP64 refinement on the matrix cores; pairs it cannot finish go to a
 //        device list
-//   FB   the listed pairs run through the FP64 pipeline (K1/K2/K3 over the list)
+//   FB   the listed pairs run through the FP64 pipeline (K1/K2/K3 over the list,
+//        whose length the kernels read on the device: no host round trip)
 // K1f..K3f and FB are chunked to bound the workspace, exactly as the FP64 path.
 #include <hip/hip_runtime.h>
 
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(
                                                      cplx* __restrict__ U, int* __restrict__ info, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, smem, vecs);
+    geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, batch, smem, vecs);
 }
 
 __global__ void __launch_bounds__(64) k1f_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(64) k1f_kernel(const cplx* __restrict__ A, con
                                                   int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplxf vecs[CF::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_MIXED, float>(A, B, W, U, info, ws, batch, smem, vecs);
+    geneig::k1_batched<kN, 1, geneig::MODE_MIXED, float>(A, B, W, U, info, ws, batch, batch, smem, vecs);
 }
 
 // K2 as the FP64 path runs it: chases of 64 pairs per wave with the rotations
@@ -74,21 +75,23 @@ constexpr unsigned kK1LdsF = geneig::k1_lds_bytes<kN, float>();
 constexpr unsigned kK1LdsD = geneig::k1_lds_bytes<kN, double>();
 constexpr unsigned kK3LdsD = geneig::k3_lds_bytes<kN, double>();
 
-__global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch, int cap) {
+__global__ void __launch_bounds__(64) k2af_kernel(int* __restrict__ info, unsigned char* ws, int batch, int cap,
+                                                   int* __restrict__ ovf) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2a_batched<kN, float>(info, ws, batch, reinterpret_cast<float*>(smem), cap);
+    geneig::k2a_batched<kN, float>(info, ws, batch, batch, reinterpret_cast<float*>(smem), cap, ovf);
 }
 
 __global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2bf_kernel(const int* __restrict__ info,
                                                                                  unsigned char* ws, int batch) {
     __shared__ cplxf buf[geneig::k2b_buf_entries<kN, float>()];
-    geneig::k2b_batched<kN, float>(info, ws, batch, buf);
+    geneig::k2b_batched<kN, float>(info, ws, batch, batch, buf);
 }
 
 __global__ void __launch_bounds__(64 * CF::R, GENEIG_K2F_WAVES_PER_EU) k2xf_kernel(int* __restrict__ info,
-                                                                                 unsigned char* ws, int batch) {
+                                                                                 unsigned char* ws, int chunk,
+                                                                                 const int* __restrict__ ovf) {
     __shared__ cplxf rotlog[CF::R * CF::K2_VEC];
-    geneig::k2x_batched<kN, float>(info, ws, batch, rotlog);
+    geneig::k2x_batched<kN, float>(info, ws, chunk, rotlog, ovf);
 }
 
 __global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3f_kernel(const cplx* __restrict__ B, double* __restrict__ W,
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3f_kernel(const 
                                                   unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplxf vecs[CF::VEC];
-    geneig::k3_batched<kN, 1, geneig::MODE_MIXED, float>(B, W, U, info, ws, batch, smem, vecs);
+    geneig::k3_batched<kN, 1, geneig::MODE_MIXED, float>(B, W, U, info, ws, batch, batch, smem, vecs);
 }
 
 __global__ void __launch_bounds__(64 * RC::NT, GENEIG_REFINE_WAVES_PER_EU) refine_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
@@ -108,47 +111,61 @@ __global__ void __launch_bounds__(64 * RC::NT, GENEIG_REFINE_WAVES_PER_EU) refin
     geneig::refine_batched<kN>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem, first);
 }
 
-// The FP64 pipeline over a list of pairs (the fallback).
+// The FP64 pipeline over the fallback list: round `off` covers entries off ..
+// off + chunk - 1 of the list, whose length the kernels read on the device
+// (dispatch_slots), so the host never waits for it.
 __global__ void __launch_bounds__(64) k1l_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                   double* __restrict__ W, cplx* __restrict__ U,
-                                                  int* __restrict__ info, unsigned char* ws, int count,
-                                                  const int* __restrict__ list) {
+                                                  int* __restrict__ info, unsigned char* ws, int chunk,
+                                                  const int* __restrict__ list, const int* __restrict__ dcount,
+                                                  int off) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_FULL, double>(A, B, W, U, info, ws, count, smem, vecs, list);
+    geneig::k1_batched<kN, 1, geneig::MODE_FULL, double, true>(
+        A, B, W, U, info, ws,
+        geneig::dispatch_slots(0, dcount, off, chunk), chunk, smem,
+                                                         vecs, list + off);
 }
 
-__global__ void __launch_bounds__(64) k2al_kernel(int* __restrict__ info, unsigned char* ws, int count,
-                                                   const int* __restrict__ list, int cap) {
+__global__ void __launch_bounds__(64) k2al_kernel(int* __restrict__ info, unsigned char* ws, int chunk,
+                                                   const int* __restrict__ list, const int* __restrict__ dcount,
+                                                   int off, int cap, int* __restrict__ ovf) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::k2a_batched<kN, double>(info, ws, count, reinterpret_cast<double*>(smem), cap, list);
+    geneig::k2a_batched<kN, double, true>(info, ws, geneig::dispatch_slots(0, dcount, off, chunk), chunk,
+                                    reinterpret_cast<double*>(smem), cap, ovf, list + off);
 }
 
 __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2B_WAVES_PER_EU) k2bl_kernel(const int* __restrict__ info,
-                                                                                 unsigned char* ws, int count,
-                                                                                 const int* __restrict__ list) {
+                                                                                 unsigned char* ws, int chunk,
+                                                                                 const int* __restrict__ list,
+                                                                                 const int* __restrict__ dcount,
+                                                                                 int off) {
     __shared__ cplx buf[geneig::k2b_buf_entries<kN, double>()];
-    geneig::k2b_batched<kN, double>(info, ws, count, buf, list);
+    geneig::k2b_batched<kN, double, true>(info, ws, geneig::dispatch_slots(0, dcount, off, chunk), chunk, buf, list + off);
 }
 
 __global__ void __launch_bounds__(64 * CD::R, GENEIG_K2_WAVES_PER_EU) k2xl_kernel(int* __restrict__ info,
-                                                                                unsigned char* ws, int count,
-                                                                                const int* __restrict__ list) {
+                                                                                unsigned char* ws, int chunk,
+                                                                                const int* __restrict__ list,
+                                                                                int off,
+                                                                                const int* __restrict__ ovf) {
     __shared__ cplx rotlog[CD::R * CD::K2_VEC];
-    geneig::k2x_batched<kN, double>(info, ws, count, rotlog, list);
+    geneig::k2x_batched<kN, double>(info, ws, chunk, rotlog, ovf, list + off);
 }
 
 __global__ void __launch_bounds__(64, GENEIG_K3_WAVES_PER_EU) k3l_kernel(const cplx* __restrict__ B, double* __restrict__ W,
                                                   cplx* __restrict__ U, const int* __restrict__ info,
-                                                  unsigned char* ws, int count, const int* __restrict__ list) {
+                                                  unsigned char* ws, int chunk, const int* __restrict__ list,
+                                                  const int* __restrict__ dcount, int off) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k3_batched<kN, 1, geneig::MODE_FULL, double>(B, W, U, info, ws, count, smem, vecs, list);
+    geneig::k3_batched<kN, 1, geneig::MODE_FULL, double, true>(B, W, U, info, ws,
+                                                         geneig::dispatch_slots(0, dcount, off, chunk), chunk, smem,
+                                                         vecs, list + off);
 }
 
-// Clears the fallback counter with a kernel store, so a captured sequence
-// carries no memset node.
-__global__ void zero_kernel(int* p) { *p = 0; }
+// Clears n ints with kernel stores, so a captured sequence carries no memset node.
+__global__ void zero_kernel(int* p, int n) { geneig::zero_ints(p, n); }
 
 std::mutex g_attr_mu;
 std::set<const void*> g_attr_done;
@@ -183,98 +200,117 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     if (maxit <= 0) maxit = kMaxIter;
     if (!(tol > 0.0)) tol = kTol;
 
-    // workspace: [fallback counter + list | solver chunk area]
-    const size_t list_bytes = ((sizeof(int) * ((size_t)batch + 1) + 255) / 256) * 256;
-    const int chunk_f = geneig_host::chunk_pairs(batch, WF::PAIR_BYTES);
-    const int chunk_d = geneig_host::chunk_pairs(batch, WD::PAIR_BYTES);
-    // a batch that fits one complex64 chunk runs as kMixedParts concurrent
-    // parts, each with its own region of the solver area and its own stream
-    const int parts = geneig_host::part_count(batch, chunk_f, kMixedParts);
-    const size_t region = (((size_t)((batch + parts - 1) / parts)) * WF::PAIR_BYTES + 255) / 256 * 256;
-    const size_t solver_bytes = std::max(parts > 1 ? region * parts : (size_t)chunk_f * WF::PAIR_BYTES,
-                                         (size_t)chunk_d * WD::PAIR_BYTES);
+    // workspace: [fallback counter + list | solver area]; a batch that fits
+    // one complex64 chunk runs as kMixedParts concurrent parts, each with its
+    // own region of the solver area and its own stream
+    const geneig_host::MixedLayout L = geneig_host::mixed_n54_layout(batch);
     void* ws = nullptr;
-    RETURN_IF(geneig_host::workspace(stream, list_bytes + solver_bytes, &ws));
+    RETURN_IF(geneig_host::workspace(stream, L.bytes, &ws));
     int* fb_count = static_cast<int*>(ws);
     int* fb_list = fb_count + 1;
-    unsigned char* solver = static_cast<unsigned char*>(ws) + list_bytes;
+    unsigned char* solver = static_cast<unsigned char*>(ws) + L.list_bytes;
     const size_t mat = (size_t)kN * kN;
 
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(1), 0, stream, fb_count);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, fb_count, 1);
     RETURN_IF(hipGetLastError());
-    // the complex64 stages for pairs [off, off + P) with solver area w on stream st
+    // the complex64 stages for pairs [off, off + P) with solver region w on stream st
     auto bulk = [&](int off, int P, unsigned char* w, hipStream_t st) -> hipError_t {
+        int* ovf = geneig_host::region_ctl(w, P, WF::PAIR_BYTES);
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ovf, 1);
+        RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, st, A + mat * off, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, st, info + off, w, P,
-                           geneig_host::ql_log_entries());
+                           geneig_host::ql_log_entries(), ovf);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2bf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, st, info + off, w, P);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)P), dim3(64 * CF::R), 0, st, info + off, w, P);
+        hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)std::min(P, geneig::kListGridK2x)), dim3(64 * CF::R), 0, st,
+                           info + off, w, P, (const int*)ovf);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, st, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, w, P);
         return hipGetLastError();
     };
-    if (parts > 1) {
+    if (L.parts > 1) {
         hipStream_t subs[geneig_host::kMaxParts];
-        RETURN_IF(geneig_host::fork_streams(stream, parts, subs));
+        RETURN_IF(geneig_host::fork_streams(stream, L.parts, subs));
         hipError_t e = hipSuccess;
-        for (int i = 0; i < parts && e == hipSuccess; ++i) {
-            const int off = (int)((long long)batch * i / parts);
-            const int P = (int)((long long)batch * (i + 1) / parts) - off;
+        for (int i = 0; i < L.parts && e == hipSuccess; ++i) {
+            const int off = (int)((long long)batch * i / L.parts);
+            const int P = (int)((long long)batch * (i + 1) / L.parts) - off;
             hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), subs[i],
                                B + mat * off, W + (size_t)kN * off, U + mat * off, info + off, P);
             e = hipGetLastError();
-            if (e == hipSuccess) e = bulk(off, P, solver + region * i, subs[i]);
+            if (e == hipSuccess) e = bulk(off, P, solver + L.region_f * i, subs[i]);
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(refine_kernel, dim3((unsigned)P), dim3(64 * RC::NT), RC::LDS, subs[i], A, B, W, U,
                                    info, fb_list, fb_count, iters, off + P, maxit, tol, off);
                 e = hipGetLastError();
             }
         }
-        const hipError_t j = geneig_host::join_streams(stream, parts, subs);
+        const hipError_t j = geneig_host::join_streams(stream, L.parts, subs);
         RETURN_IF(e);
         RETURN_IF(j);
     } else {
         hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)batch), dim3(64), geneig::chol64_lds_bytes<kN>(), stream, B,
                            W, U, info, batch);
         RETURN_IF(hipGetLastError());
-        for (int off = 0; off < batch; off += chunk_f) RETURN_IF(bulk(off, std::min(chunk_f, batch - off), solver, stream));
+        for (int off = 0; off < batch; off += L.chunk_f)
+            RETURN_IF(bulk(off, std::min(L.chunk_f, batch - off), solver, stream));
         hipLaunchKernelGGL(refine_kernel, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, W, U, info,
                            fb_list, fb_count, iters, batch, maxit, tol, 0);
         RETURN_IF(hipGetLastError());
     }
 
-    int count = 0;
-    RETURN_IF(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
-    RETURN_IF(hipStreamSynchronize(stream));
-    if (count < 0 || count > batch) return hipErrorUnknown;
-    if (h_fallback) *h_fallback = count;
-    for (int off = 0; off < count; off += chunk_d) {
-        const int P = std::min(chunk_d, count - off);
-        hipLaunchKernelGGL(k1l_kernel, dim3((unsigned)P), dim3(64), kK1LdsD, stream, A, B, W, U, info, solver, P,
-                           fb_list + off);
+    // The FP64 re-solve of the listed pairs, in rounds of chunk_d list
+    // entries; every dispatch reads the list's length on the device and walks
+    // it with a fixed grid (an empty list costs a few microseconds), so the
+    // launch never waits for the host.
+    int* fovf = geneig_host::region_ctl(solver, L.chunk_d, WD::PAIR_BYTES);
+    const int cd = L.chunk_d;
+    const unsigned g1 = (unsigned)std::min(cd, geneig::kListGridK1);
+    const unsigned g2a = (unsigned)std::min((cd + 63) / 64, geneig::kListGridK2a);
+    const unsigned g2x = (unsigned)std::min(cd, geneig::kListGridK2x);
+    const int* lst = fb_list;
+    const int* cnt = fb_count;
+    for (int r = 0; r < L.rounds; ++r) {
+        const int off = r * cd;
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, fovf, 1);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2al_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsD, stream, info, solver,
-                           P, fb_list + off, geneig_host::ql_log_entries());
+        hipLaunchKernelGGL(k1l_kernel, dim3(g1), dim3(64), kK1LdsD, stream, A, B, W, U, info, solver, cd, lst, cnt,
+                           off);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2bl_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
-                           fb_list + off);
+        hipLaunchKernelGGL(k2al_kernel, dim3(g2a), dim3(64), kK2aLdsD, stream, info, solver, cd, lst, cnt, off,
+                           geneig_host::ql_log_entries(), fovf);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k2xl_kernel, dim3((unsigned)P), dim3(64 * CD::R), 0, stream, info, solver, P,
-                           fb_list + off);
+        hipLaunchKernelGGL(k2bl_kernel, dim3(g1), dim3(64 * CD::R), 0, stream, info, solver, cd, lst, cnt, off);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3l_kernel, dim3((unsigned)P), dim3(64), kK3LdsD, stream, B, W, U, info, solver, P,
-                           fb_list + off);
+        hipLaunchKernelGGL(k2xl_kernel, dim3(g2x), dim3(64 * CD::R), 0, stream, info, solver, cd, lst, off,
+                           (const int*)fovf);
         RETURN_IF(hipGetLastError());
+        hipLaunchKernelGGL(k3l_kernel, dim3(g1), dim3(64), kK3LdsD, stream, B, W, U, info, solver, cd, lst, cnt, off);
+        RETURN_IF(hipGetLastError());
+    }
+    if (h_fallback) {
+        // reported on request only: this is the launch's one wait for the device
+        int count = 0;
+        RETURN_IF(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+        RETURN_IF(hipStreamSynchronize(stream));
+        if (count < 0 || count > batch) return hipErrorUnknown;
+        *h_fallback = count;
     }
     return hipSuccess;
 }
 
 }  // namespace
+
+namespace geneig_host {
+MixedLayout mixed_n54_layout(int batch) {
+    return mixed_layout(batch, WF::PAIR_BYTES, WD::PAIR_BYTES, kMixedParts);
+}
+}  // namespace geneig_host
 
 extern "C" {
 

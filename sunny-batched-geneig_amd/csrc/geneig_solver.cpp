@@ -56,6 +56,10 @@ int geneig_host::fail(int code, const char* fmt, ...) {
 namespace {
 
 using geneig_host::fail;
+// Grids of the dispatches that walk device-built lists (the same values as
+// kListGrid* of geneig_device.inc, which the host part of this file does not
+// include).
+constexpr int kListGridK1 = 4096, kListGridK2a = 512, kListGridK2x = 1024;
 #define HIP_TRY(expr) GENEIG_HIP_TRY(expr)
 
 // ---- content hashes (FNV-1a 64, the reference's cache fingerprint,
@@ -206,7 +210,7 @@ struct Module {
     // refinement, FP64 K1/K2/K3 over a pair list, counter clear
     hipFunction_t fc = nullptr, f1f = nullptr, f2af = nullptr, f2bf = nullptr, f2xf = nullptr, f3f = nullptr;
     hipFunction_t fr = nullptr, f1l = nullptr, f2al = nullptr, f2bl = nullptr, f2xl = nullptr, f3l = nullptr;
-    hipFunction_t fz = nullptr;
+    hipFunction_t fz = nullptr;            // clears device counters (both variants)
     unsigned f_lds_per_pair = 0, f_k2_threads = 0, refine_lds = 0, refine_threads = 0;
     unsigned long long f_ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
@@ -238,7 +242,6 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
         HIP_TRY(hipModuleGetFunction(&m->f2bl, m->mod, "geneig_k2bl"));
         HIP_TRY(hipModuleGetFunction(&m->f2xl, m->mod, "geneig_k2xl"));
         HIP_TRY(hipModuleGetFunction(&m->f3l, m->mod, "geneig_k3l"));
-        HIP_TRY(hipModuleGetFunction(&m->fz, m->mod, "geneig_zero"));
     } else {
         HIP_TRY(hipModuleGetFunction(&m->fn, m->mod, "geneig_k1"));
         HIP_TRY(hipModuleGetFunction(&m->fn2a, m->mod, "geneig_k2a"));
@@ -246,6 +249,7 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
         HIP_TRY(hipModuleGetFunction(&m->fn2x, m->mod, "geneig_k2x"));
         HIP_TRY(hipModuleGetFunction(&m->fn3, m->mod, "geneig_k3"));
     }
+    HIP_TRY(hipModuleGetFunction(&m->fz, m->mod, "geneig_zero"));
     auto read_global = [&](const char* name, void* dst, size_t want) -> int {
         hipDeviceptr_t ptr = nullptr;
         size_t size = 0;
@@ -462,32 +466,50 @@ struct geneig_solver {
         return best;
     }
 
+    // Workspace layouts of a launch of `batch` pairs (shared with reserve).
+    geneig_host::PlainLayout plain_layout_of(const Module& m, int batch) const {
+        return geneig_host::plain_layout(batch, m.ws_pair_bytes, 1);
+    }
+    geneig_host::MixedLayout mixed_layout_of(const Module& m, int batch) const {
+        return geneig_host::mixed_layout(batch, m.f_ws_pair_bytes, m.ws_pair_bytes, 1);
+    }
+    size_t workspace_bytes(int batch) const {
+        if (batch <= 0) return 0;
+        return variant == 1 ? mixed_layout_of(*module, batch).bytes : plain_layout_of(*module, batch).bytes;
+    }
+
     // K1 -> K2 -> K3 over the batch, chunked to bound the per-stream workspace.
     int launch_raw(const Module& m, int b, const void* H, const void* S, double* W, void* U, int* info,
                    int batch, hipStream_t stream) {
         if (batch <= 0) return GENEIG_E_OK;
-        const int chunk = geneig_host::chunk_pairs(batch, m.ws_pair_bytes);
+        const geneig_host::PlainLayout L = plain_layout_of(m, batch);
         void* ws = nullptr;
-        HIP_TRY(geneig_host::workspace(stream, (size_t)chunk * m.ws_pair_bytes, &ws));
+        HIP_TRY(geneig_host::workspace(stream, L.bytes, &ws));
         const size_t mat = (size_t)n * n * 16;
         const unsigned bytes = m.lds_per_pair * (unsigned)b;
-        for (int off = 0; off < batch; off += chunk) {
-            int P = std::min(chunk, batch - off);
+        for (int off = 0; off < batch; off += L.chunk) {
+            int P = std::min(L.chunk, batch - off);
             const void* h = static_cast<const char*>(H) + mat * off;
             const void* sp = static_cast<const char*>(S) + mat * off;
             double* w = W + (size_t)n * off;
             void* u = static_cast<char*>(U) + mat * off;
             int* inf = info + off;
+            int* ovf = geneig_host::region_ctl(ws, P, m.ws_pair_bytes);
+            int one = 1;
+            void* az[] = {&ovf, &one};
+            HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, az, nullptr));
             void* a1[] = {&h, &sp, &w, &u, &inf, &ws, &P};
             const unsigned g13 = (unsigned)((P + b - 1) / b);
             HIP_TRY(hipModuleLaunchKernel(m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1, nullptr));
-            void* a2[] = {&inf, &ws, &P};
             int cap = geneig_host::ql_log_entries();
-            void* a2a[] = {&inf, &ws, &P, &cap};
+            void* a2a[] = {&inf, &ws, &P, &cap, &ovf};
             HIP_TRY(hipModuleLaunchKernel(m.fn2a, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a,
                                           nullptr));
+            void* a2[] = {&inf, &ws, &P};
             HIP_TRY(hipModuleLaunchKernel(m.fn2b, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
-            HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            void* a2x[] = {&inf, &ws, &P, &ovf};
+            HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)std::min(P, kListGridK2x), 1, 1, m.k2_threads, 1, 1, 0,
+                                          stream, a2x, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
             HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, m.k3_lds * (unsigned)b, stream, a3,
                                           nullptr));
@@ -497,69 +519,81 @@ struct geneig_solver {
 
     // Precision form over the batch (the sequence of geneig_mixed.hip with
     // the module's kernels): FP64 Cholesky, complex64 K1/K2/K3 per chunk,
-    // FP64 refinement, then the FP64 pipeline over the pairs it lists.
-    // Synchronizes `stream` once, to read the fallback count.
+    // FP64 refinement, then the FP64 pipeline over the pairs it lists, whose
+    // number the kernels read on the device.  Waits for the device only when
+    // h_fallback asks for that number.
     int launch_mixed(const Module& m, const void* H, const void* S, double* W, void* U, int* info, int batch,
                      int maxit, double tol, int* iters, int* h_fallback, hipStream_t stream) {
         if (h_fallback) *h_fallback = 0;
         if (batch <= 0) return GENEIG_E_OK;
         if (maxit <= 0) maxit = 4;
         if (!(tol > 0.0)) tol = 1e-7;
-        const size_t list_bytes = ((sizeof(int) * ((size_t)batch + 1) + 255) / 256) * 256;
-        const int chunk_f = geneig_host::chunk_pairs(batch, m.f_ws_pair_bytes);
-        const int chunk_d = geneig_host::chunk_pairs(batch, m.ws_pair_bytes);
-        const size_t solver_bytes = std::max((size_t)((size_t)chunk_f * m.f_ws_pair_bytes), (size_t)((size_t)chunk_d * m.ws_pair_bytes));
+        const geneig_host::MixedLayout L = mixed_layout_of(m, batch);
         void* ws = nullptr;
-        HIP_TRY(geneig_host::workspace(stream, list_bytes + solver_bytes, &ws));
+        HIP_TRY(geneig_host::workspace(stream, L.bytes, &ws));
         int* fb_count = static_cast<int*>(ws);
         int* fb_list = fb_count + 1;
-        void* solver = static_cast<char*>(ws) + list_bytes;
+        void* solver = static_cast<char*>(ws) + L.list_bytes;
         const size_t mat = (size_t)n * n * 16;
-        void* az[] = {&fb_count};
-        HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 1, 1, 1, 0, stream, az, nullptr));
+        int one = 1;
+        void* az[] = {&fb_count, &one};
+        HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, az, nullptr));
         void* ac[] = {&S, &W, &U, &info, &batch};
         HIP_TRY(hipModuleLaunchKernel(m.fc, (unsigned)batch, 1, 1, 64, 1, 1, m.chol64_lds, stream, ac, nullptr));
-        for (int off = 0; off < batch; off += chunk_f) {
-            int P = std::min(chunk_f, batch - off);
+        for (int off = 0; off < batch; off += L.chunk_f) {
+            int P = std::min(L.chunk_f, batch - off);
             const void* h = static_cast<const char*>(H) + mat * off;
             const void* sp = static_cast<const char*>(S) + mat * off;
             double* w = W + (size_t)n * off;
             void* u = static_cast<char*>(U) + mat * off;
             int* inf = info + off;
+            int* ovf = geneig_host::region_ctl(solver, P, m.f_ws_pair_bytes);
+            void* azo[] = {&ovf, &one};
+            HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, azo, nullptr));
             void* a1[] = {&h, &sp, &w, &u, &inf, &solver, &P};
             HIP_TRY(hipModuleLaunchKernel(m.f1f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a1, nullptr));
-            void* a2[] = {&inf, &solver, &P};
             int cap = geneig_host::ql_log_entries();
-            void* a2a[] = {&inf, &solver, &P, &cap};
+            void* a2a[] = {&inf, &solver, &P, &cap, &ovf};
             HIP_TRY(hipModuleLaunchKernel(m.f2af, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.f_k2a_lds, stream, a2a,
                                           nullptr));
+            void* a2[] = {&inf, &solver, &P};
             HIP_TRY(hipModuleLaunchKernel(m.f2bf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
-            HIP_TRY(hipModuleLaunchKernel(m.f2xf, (unsigned)P, 1, 1, m.f_k2_threads, 1, 1, 0, stream, a2, nullptr));
+            void* a2x[] = {&inf, &solver, &P, &ovf};
+            HIP_TRY(hipModuleLaunchKernel(m.f2xf, (unsigned)std::min(P, kListGridK2x), 1, 1, m.f_k2_threads, 1, 1, 0,
+                                          stream, a2x, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &solver, &P};
             HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, 64, 1, 1, m.f_k3_lds, stream, a3, nullptr));
         }
         void* ar[] = {&H, &S, &W, &U, &info, &fb_list, &fb_count, &iters, &batch, &maxit, &tol};
         HIP_TRY(hipModuleLaunchKernel(m.fr, (unsigned)batch, 1, 1, m.refine_threads, 1, 1, m.refine_lds, stream, ar,
                                       nullptr));
-        int count = 0;
-        HIP_TRY(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        if (count < 0 || count > batch) return fail(GENEIG_E_HIP, "fallback count %d out of range", count);
-        if (h_fallback) *h_fallback = count;
-        for (int off = 0; off < count; off += chunk_d) {
-            int P = std::min(chunk_d, count - off);
-            const int* lst = fb_list + off;
-            void* a1[] = {&H, &S, &W, &U, &info, &solver, &P, &lst};
-            HIP_TRY(hipModuleLaunchKernel(m.f1l, (unsigned)P, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a1, nullptr));
-            void* a2[] = {&info, &solver, &P, &lst};
+        int cd = L.chunk_d;
+        int* fovf = geneig_host::region_ctl(solver, cd, m.ws_pair_bytes);
+        const unsigned g1 = (unsigned)std::min(cd, kListGridK1);
+        const unsigned g2a = (unsigned)std::min((cd + 63) / 64, kListGridK2a);
+        const unsigned g2x = (unsigned)std::min(cd, kListGridK2x);
+        for (int r = 0; r < L.rounds; ++r) {
+            int off = r * cd;
+            void* azf[] = {&fovf, &one};
+            HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, azf, nullptr));
+            void* a1[] = {&H, &S, &W, &U, &info, &solver, &cd, &fb_list, &fb_count, &off};
+            HIP_TRY(hipModuleLaunchKernel(m.f1l, g1, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a1, nullptr));
             int cap = geneig_host::ql_log_entries();
-            void* a2a[] = {&info, &solver, &P, &lst, &cap};
-            HIP_TRY(hipModuleLaunchKernel(m.f2al, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a,
-                                          nullptr));
-            HIP_TRY(hipModuleLaunchKernel(m.f2bl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
-            HIP_TRY(hipModuleLaunchKernel(m.f2xl, (unsigned)P, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
-            void* a3[] = {&S, &W, &U, &info, &solver, &P, &lst};
-            HIP_TRY(hipModuleLaunchKernel(m.f3l, (unsigned)P, 1, 1, 64, 1, 1, m.k3_lds, stream, a3, nullptr));
+            void* a2a[] = {&info, &solver, &cd, &fb_list, &fb_count, &off, &cap, &fovf};
+            HIP_TRY(hipModuleLaunchKernel(m.f2al, g2a, 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a, nullptr));
+            void* a2[] = {&info, &solver, &cd, &fb_list, &fb_count, &off};
+            HIP_TRY(hipModuleLaunchKernel(m.f2bl, g1, 1, 1, m.k2_threads, 1, 1, 0, stream, a2, nullptr));
+            void* a2x[] = {&info, &solver, &cd, &fb_list, &off, &fovf};
+            HIP_TRY(hipModuleLaunchKernel(m.f2xl, g2x, 1, 1, m.k2_threads, 1, 1, 0, stream, a2x, nullptr));
+            void* a3[] = {&S, &W, &U, &info, &solver, &cd, &fb_list, &fb_count, &off};
+            HIP_TRY(hipModuleLaunchKernel(m.f3l, g1, 1, 1, 64, 1, 1, m.k3_lds, stream, a3, nullptr));
+        }
+        if (h_fallback) {
+            int count = 0;
+            HIP_TRY(hipMemcpyAsync(&count, fb_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            if (count < 0 || count > batch) return fail(GENEIG_E_HIP, "fallback count %d out of range", count);
+            *h_fallback = count;
         }
         return GENEIG_E_OK;
     }
@@ -910,14 +944,13 @@ int geneig_solver_launch_chunked_host(geneig_solver_t s, const void* h_H, const 
 int geneig_solver_reserve(geneig_solver_t s, int batch_size, hipStream_t stream) {
     if (!s) return fail(GENEIG_E_ARG, "null solver");
     if (batch_size <= 0) return GENEIG_E_OK;
-    const Module& m = *s->module;
-    size_t bytes = (size_t)geneig_host::chunk_pairs(batch_size, m.ws_pair_bytes) * m.ws_pair_bytes;
-    if (s->variant == 1)
-        bytes = ((sizeof(int) * ((size_t)batch_size + 1) + 255) / 256) * 256 +
-                std::max(bytes, (size_t)((size_t)geneig_host::chunk_pairs(batch_size, m.f_ws_pair_bytes) * m.f_ws_pair_bytes));
     void* ws = nullptr;
-    HIP_TRY(geneig_host::workspace(stream, bytes, &ws));
+    HIP_TRY(geneig_host::workspace(stream, s->workspace_bytes(batch_size), &ws));
     return GENEIG_E_OK;
+}
+
+size_t geneig_solver_workspace_bytes(geneig_solver_t s, int batch_size) {
+    return s ? s->workspace_bytes(batch_size) : 0;
 }
 
 int geneig_solver_matrix_size(geneig_solver_t s) { return s ? s->n : 0; }

@@ -7,6 +7,7 @@
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 
 namespace geneig_host {
 
@@ -15,6 +16,8 @@ namespace {
 struct Buf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool captured = false;          // the stream was seen under capture
+    std::vector<std::pair<void*, size_t>> retired;   // outgrown buffers a captured graph may still use
 };
 
 std::mutex g_mu;
@@ -47,6 +50,48 @@ int ql_log_entries() {
     if (!e || !*e) return INT_MAX;
     const long v = std::atol(e);
     return v > 0 && v < INT_MAX ? (int)v : INT_MAX;
+}
+
+int part_count(int batch, int chunk, int want) {
+    if (want < 1) return 1;
+    if (want > kMaxParts) want = kMaxParts;
+    if (chunk < batch) return 1;
+    while (want > 1 && batch / want < kMinPartPairs) --want;
+    return want;
+}
+
+PlainLayout plain_layout(int batch, unsigned long long pair_bytes, int want_parts) {
+    PlainLayout L;
+    if (batch <= 0) return L;
+    L.chunk = chunk_pairs(batch, pair_bytes);
+    L.parts = part_count(batch, L.chunk, want_parts);
+    if (L.parts > 1) {
+        L.part_slots = (batch + L.parts - 1) / L.parts;
+        L.region = region_bytes(L.part_slots, pair_bytes);
+        L.bytes = L.region * L.parts;
+    } else {
+        L.part_slots = L.chunk;
+        L.region = region_bytes(L.chunk, pair_bytes);
+        L.bytes = L.region;
+    }
+    return L;
+}
+
+MixedLayout mixed_layout(int batch, unsigned long long pair_bytes_f, unsigned long long pair_bytes_d,
+                         int want_parts, unsigned long long extra_pair_bytes) {
+    MixedLayout L;
+    if (batch <= 0) return L;
+    const unsigned long long pf = pair_bytes_f + extra_pair_bytes;
+    L.list_bytes = round256(sizeof(int) * ((size_t)batch + 1));
+    L.chunk_f = chunk_pairs(batch, pf);
+    L.chunk_d = chunk_pairs(batch, pair_bytes_d);
+    L.rounds = (batch + L.chunk_d - 1) / L.chunk_d;
+    L.parts = part_count(batch, L.chunk_f, want_parts);
+    L.part_slots = L.parts > 1 ? (batch + L.parts - 1) / L.parts : L.chunk_f;
+    L.region_f = region_bytes(L.part_slots, pf);
+    L.region_d = region_bytes(L.chunk_d, pair_bytes_d);
+    L.bytes = L.list_bytes + std::max(L.region_f * L.parts, L.region_d);
+    return L;
 }
 
 namespace {
@@ -95,6 +140,14 @@ hipError_t parts_of(hipStream_t stream, int parts, Parts** out) {
 
 }  // namespace
 
+hipError_t prepare_parts(hipStream_t stream, int parts) {
+    if (parts <= 1) return hipSuccess;
+    if (parts > kMaxParts) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lk(g_parts_mu);
+    Parts* p = nullptr;
+    return parts_of(stream, parts, &p);
+}
+
 hipError_t fork_streams(hipStream_t stream, int parts, hipStream_t* subs) {
     if (parts < 1 || parts > kMaxParts) return hipErrorInvalidValue;
     std::lock_guard<std::mutex> lk(g_parts_mu);
@@ -127,26 +180,28 @@ hipError_t join_streams(hipStream_t stream, int parts, const hipStream_t* subs) 
     return hipSuccess;
 }
 
-int part_count(int batch, int chunk, int want) {
-    if (want < 1) return 1;
-    if (want > kMaxParts) want = kMaxParts;
-    if (chunk < batch) return 1;
-    while (want > 1 && batch / want < kMinPartPairs) --want;
-    return want;
-}
-
 hipError_t workspace(hipStream_t stream, size_t bytes, void** out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    e = hipStreamIsCapturing(stream, &cs);
+    if (e != hipSuccess) return e;
+    const bool capturing = (cs == hipStreamCaptureStatusActive);
     std::lock_guard<std::mutex> lk(g_mu);
     Buf& b = g_bufs[{dev, stream}];
+    if (capturing) b.captured = true;
     if (b.bytes < bytes) {
+        if (capturing) return hipErrorStreamCaptureUnsupported;   // reserve before capturing
         if (b.ptr) {
-            // the old buffer may still be read by work queued on this stream
-            e = hipStreamSynchronize(stream);
-            if (e != hipSuccess) return e;
-            (void)hipFree(b.ptr);
+            if (b.captured) {
+                b.retired.push_back({b.ptr, b.bytes});
+            } else {
+                // the old buffer may still be read by work queued on this stream
+                e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) return e;
+                (void)hipFree(b.ptr);
+            }
             b.ptr = nullptr;
             b.bytes = 0;
         }
@@ -158,6 +213,27 @@ hipError_t workspace(hipStream_t stream, size_t bytes, void** out) {
         b.bytes = bytes;
     }
     *out = b.ptr;
+    return hipSuccess;
+}
+
+hipError_t release(hipStream_t stream, size_t* freed) {
+    if (freed) *freed = 0;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_bufs.find({dev, stream});
+    if (it == g_bufs.end()) return hipSuccess;
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    Buf& b = it->second;
+    if (freed) *freed = b.bytes;
+    if (b.ptr) (void)hipFree(b.ptr);
+    for (auto& r : b.retired) {
+        (void)hipFree(r.first);
+        if (freed) *freed += r.second;
+    }
+    g_bufs.erase(it);
     return hipSuccess;
 }
 

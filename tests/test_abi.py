@@ -112,3 +112,41 @@ def test_product_does_not_touch_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".inc", ".h", "Makefile")) and "_build" not in dirpath:
                 txt = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in txt.lower(), f
+
+
+def _r256(b):
+    return (b + 255) // 256 * 256
+
+
+def _pair_bytes(n, real):
+    """Workspace bytes per pair of the solve stages: d, e (n reals each), Q
+    (n^2 complex) and the slot that holds the QL rotation log (guard 8 +
+    capacity 3n^2/2 + 64 + read-ahead 64 ceil(n/64) entries of two reals)
+    and later Z (n^2 reals)."""
+    log = 2 * (8 + 3 * n * n // 2 + 64 + 64 * ((n + 63) // 64))
+    return real * (2 * n + 2 * n * n + max(log, n * n))
+
+
+def _region(slots, pb):
+    return _r256(slots * pb) + _r256(4 * (slots + 1))
+
+
+@pytest.mark.parametrize("batch", [1, 37, 8191, 16385, 24575, 24576, 131071])
+def test_n54_workspace_sizing(geneig, batch):
+    """The bytes a batched n = 54 launch reserves: one region per concurrent
+    part (three when every part keeps >= 8192 pairs, else two, else one
+    chunked region), each holding its slots' stage arrays and the overflow
+    list; the precision form adds the fallback list and takes the larger of
+    its complex64 regions and the FP64 re-solve region.  Odd batches round
+    the part size up, so every part's pairs fit its region."""
+    pd, pf = _pair_bytes(54, 8), _pair_bytes(54, 4)
+    parts = 3 if batch // 3 >= 8192 else (2 if batch // 2 >= 8192 else 1)
+    slots = -(-batch // parts)
+    want0 = parts * _region(slots, pd)
+    assert geneig.n54_workspace_bytes(batch, 0) == want0
+    for i in range(parts):   # the parts the launcher cuts
+        P = batch * (i + 1) // parts - batch * i // parts
+        assert _region(P, pd) <= _region(slots, pd)
+    want1 = _r256(4 * (batch + 1)) + max(parts * _region(slots, pf), _region(batch, pd))
+    assert geneig.n54_workspace_bytes(batch, 1) == want1
+    assert geneig.n54_workspace_bytes(0, 0) == 0 and geneig.n54_workspace_bytes(-5, 1) == 0

@@ -219,3 +219,68 @@ def test_mixed_concurrent_parts_match_single_launches(geneig, oracle, device):
     assert fb == fb2
     assert torch.equal(info, i2) and torch.equal(iters, it2) and int(iters.min()) >= 1
     assert torch.equal(W, W2) and torch.equal(U, U2)
+
+
+def test_mixed_fallback_parts_all_pairs(geneig, oracle, device):
+    """24,576 generic pairs run as three concurrent parts; with a tolerance no
+    refinement can meet, every pair is listed by three refinement dispatches
+    at once (atomics on one counter) and re-solved by the FP64 pipeline over
+    the list after the join, in the solver area the parts used.  The list's
+    length is read on the device, and the fixed grids walk it in several
+    trips.  Outputs equal the FP64 path bit for bit."""
+    import torch
+
+    total = 24576
+    assert geneig.n54_workspace_bytes(total, 1) > 0
+    A, B = geneig.gen_generic(54, total, seed=21)
+    W, U, info = geneig.empty_outputs(54, total, device)
+    iters = torch.full((total,), -1, dtype=torch.int32, device=device)
+    fb = geneig.mixed_n54_batched(A, B, W, U, info, iters=iters, max_iter=1, tol=1e-300)
+    W2, U2, i2 = geneig.empty_outputs(54, total, device)
+    geneig.full_n54_batched(A, B, W2, U2, i2)
+    torch.cuda.synchronize()
+    print("all-fallback parts: fb %d of %d" % (fb, total), flush=True)
+    assert fb == total and int((iters == 1).sum()) == total
+    assert torch.equal(info, i2) and int(info.abs().max()) == 0
+    assert torch.equal(W, W2) and torch.equal(U, U2)
+
+
+def test_mixed_graph_capture(geneig, oracle, device):
+    """The precision form reads its fallback count on the device, so with
+    report_fallback=False the launch never waits for the host and captures
+    into a graph (after n54_reserve on the capturing stream).  Two captures:
+    16,384 spin-wave pairs (two parts, no fallback) and 512 generic pairs that
+    all fall back (max_iter=1, tol=1e-300); each is replayed three times with
+    eager work and synchronizations between, and every replay equals the
+    eager launch bit for bit."""
+    import torch
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    cases = [("spin-wave", geneig.gen_spinwave(27, 16384, seed=13), {}),
+             ("all-fallback", geneig.gen_generic(54, 512, seed=14), dict(max_iter=1, tol=1e-300))]
+    for name, (A, B), kw in cases:
+        total = A.shape[0]
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            geneig.n54_reserve(total, 1, stream=s)
+            W0, U0, i0 = geneig.empty_outputs(54, total, device)
+            fb0 = geneig.mixed_n54_batched(A, B, W0, U0, i0, stream=s, **kw)
+        s.synchronize()
+        W, U, info = geneig.empty_outputs(54, total, device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            geneig.mixed_n54_batched(A, B, W, U, info, stream=s, report_fallback=False, **kw)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            W.fill_(float("nan"))
+            U.fill_(float("nan"))
+            info.fill_(-1)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(W, W0) and torch.equal(U, U0) and torch.equal(info, i0), (name, rep)
+        print("%s: fallback %d of %d, 3 replays equal" % (name, fb0, total), flush=True)
+        assert int(i0.abs().max()) == 0
+        assert fb0 == (total if kw else 0)
+        del g

@@ -345,3 +345,38 @@ def test_concurrent_parts_under_graph_capture(geneig, oracle, device):
         torch.cuda.synchronize()
         assert torch.equal(W, W0) and torch.equal(U, U0) and torch.equal(info, i0), rep
     assert int(i0.abs().max()) == 0
+
+
+def test_workspace_reserve_and_release(geneig, oracle, device):
+    """n54_reserve allocates what a launch of that batch uses (the launch
+    then grows nothing), workspace_release frees it and reports the bytes,
+    and a launch after the release allocates again and gives the same
+    results."""
+    import torch
+
+    total = 20000
+    A, B = geneig.gen_spinwave(27, total, seed=15)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    outs = []
+    with torch.cuda.stream(s):
+        for form in (0, 1):
+            geneig.n54_reserve(total, form, stream=s)
+            assert geneig.workspace_release(stream=s) == geneig.n54_workspace_bytes(total, form)
+            assert geneig.workspace_release(stream=s) == 0
+            geneig.n54_reserve(total, form, stream=s)
+            W, U, info = geneig.empty_outputs(54, total, device)
+            run = geneig.full_n54_batched if form == 0 else geneig.mixed_n54_batched
+            run(A, B, W, U, info, stream=s)
+            outs.append((W, U, info))
+            assert geneig.workspace_release(stream=s) == geneig.n54_workspace_bytes(total, form)
+    s.synchronize()
+    W2, U2, i2 = geneig.empty_outputs(54, total, device)
+    geneig.full_n54_batched(A, B, W2, U2, i2)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0][0], W2) and torch.equal(outs[0][1], U2)
+    for W, U, info in outs:
+        assert int(info.abs().max()) == 0
+    Ah, Bh = to_host(A[:32]), to_host(B[:32])
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    check_pairs(oracle, Ah, Bh, to_host(outs[1][0][:32]), to_host(outs[1][1][:32]), Wo, Xo, 54, "precision after reserve")
