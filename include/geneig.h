@@ -116,6 +116,27 @@ int geneig_n54_reserve(int batch_size, int form, hipStream_t stream);
  * `stream` afterwards.  Works for the workspaces of runtime solvers too. */
 int geneig_workspace_release(hipStream_t stream, size_t* freed_bytes);
 
+/* ======================================================================== */
+/* Allocation-free batched solve.  No counterpart in the reference: for     */
+/* callers that cannot allocate device memory (code under graph capture,    */
+/* tight serving loops, harnesses that forbid hipMalloc).                   */
+/* ======================================================================== */
+
+/* geneig_full_n54_batched_launch with its stage workspace in a fixed device
+ * array of the library (512 MiB per GPU, held from the library's first
+ * launch on that GPU until the process ends) instead of the per-stream
+ * workspace: no hipMalloc, hipFree, stream or event creation.  At most
+ * geneig_full_n54_batched_noalloc_max_batch() pairs per call (4,485); a
+ * larger batch returns hipErrorInvalidValue with the reason in
+ * geneig_last_error().  Runs on `stream` alone (no concurrent parts).
+ * Every call on one GPU shares the array: calls that overlap on one GPU
+ * (other streams, other threads) race; calls ordered on one stream do not.
+ * Outputs equal geneig_full_n54_batched_launch's bit for bit. */
+int geneig_full_n54_batched_launch_noalloc(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S,
+                                           double* d_W, hipDoubleComplex* d_U, int* d_info,
+                                           int batch_size, hipStream_t stream);
+int geneig_full_n54_batched_noalloc_max_batch(void);
+
 #ifdef __cplusplus
 }
 #endif

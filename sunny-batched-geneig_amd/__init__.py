@@ -91,6 +91,8 @@ def lib():
         "geneig_n54_workspace_bytes": (C.c_size_t, [I, I]),
         "geneig_n54_reserve": (I, [I, I, P]),
         "geneig_workspace_release": (I, [P, P]),
+        "geneig_full_n54_batched_launch_noalloc": (I, [P, P, P, P, P, I, P]),
+        "geneig_full_n54_batched_noalloc_max_batch": (I, []),
         "geneig_multi_create": (I, [P, I, P, I, I, I]),
         "geneig_multi_destroy": (None, [P]),
         "geneig_multi_launch": (I, [P, P, P, P, P, P, P, P]),
@@ -236,6 +238,22 @@ def mixed_n54_batched(H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=N
                                                      C.byref(fb) if report_fallback else None, _stream(stream)),
                "geneig_mixed_n54_batched_launch")
     return fb.value if report_fallback else None
+
+
+def full_n54_batched_noalloc(H, S, W, U, info, batch=None, stream=None):
+    """geneig_full_n54_batched_launch_noalloc: the batched FP64 solve with its workspace
+    in the library's fixed device array (no device allocation; at most
+    ``noalloc_max_batch()`` pairs; calls overlapping on one GPU race)."""
+    batch = H.shape[0] if batch is None else batch
+    _check_batch(54, batch, H, S, W, U, info)
+    rc = lib().geneig_full_n54_batched_launch_noalloc(_ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
+                                                      _stream(stream))
+    if rc != 0:
+        raise GeneigError("geneig_full_n54_batched_launch_noalloc failed: hipError %d: %s" % (rc, last_error()))
+
+
+def noalloc_max_batch():
+    return int(lib().geneig_full_n54_batched_noalloc_max_batch())
 
 
 def n54_workspace_bytes(batch, form=0):

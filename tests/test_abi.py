@@ -150,3 +150,16 @@ def test_n54_workspace_sizing(geneig, batch):
     want1 = _r256(4 * (batch + 1)) + max(parts * _region(slots, pf), _region(batch, pd))
     assert geneig.n54_workspace_bytes(batch, 1) == want1
     assert geneig.n54_workspace_bytes(0, 0) == 0 and geneig.n54_workspace_bytes(-5, 1) == 0
+
+
+def test_noalloc_refuses_oversized_batch_before_device_work(geneig):
+    """The allocation-free launch checks the batch against its fixed array
+    before any HIP call: a batch one past the maximum returns an error (here
+    with null pointers, on a host without a device) naming the limit."""
+    L = geneig.lib()
+    mx = L.geneig_full_n54_batched_noalloc_max_batch()
+    assert mx == 4485
+    rc = L.geneig_full_n54_batched_launch_noalloc(None, None, None, None, None, mx + 1, None)
+    assert rc != 0
+    assert "split the batch" in geneig.last_error() and str(mx) in geneig.last_error()
+    assert L.geneig_full_n54_batched_launch_noalloc(None, None, None, None, None, 0, None) == 0
