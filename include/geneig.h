@@ -85,6 +85,16 @@ int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleC
 unsigned int geneig_full_n54_batched_block_dim_x(void);       /* geneig_fixed.cu:495 */
 unsigned int geneig_full_n54_batched_shared_mem_bytes(void);  /* geneig_fixed.cu:496 */
 
+/* In-place batched solve with LAPACK zhegvd's storage, for callers of
+ * Sunny's hegvd_batched! (the Julia FFI, RISKS.md:23-34): on return d_H holds
+ * the B-orthonormal eigenvectors (what d_U holds after the launch above) and
+ * the lower triangle of d_S the Cholesky factor L (S = L L^H); W and info as
+ * above.  When B is not positive definite, info = n + j and the pair's
+ * d_H and W are NaN.  No separate reference entry point: the reference's
+ * batched launcher (geneig_fixed.cu:474-492) writes a separate U. */
+int geneig_full_n54_batched_inplace_launch(hipDoubleComplex* d_H, hipDoubleComplex* d_S, double* d_W,
+                                           int* d_info, int batch_size, hipStream_t stream);
+
 /* Batched finish from a given factor L (lower, zero upper) and reduced M:
  * heev(M) then U = L^{-H} V.
  * Replaces geneig_finish_n54_launch, src/geneig_fixed.cu:569-587. */

@@ -49,6 +49,12 @@ void geneig_solver_destroy(geneig_solver_t s);
  * reference, a partial last workgroup is handled (no padding needed). */
 int geneig_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W,
                          void* d_U, int* d_info, int batch_size, hipStream_t stream);
+/* launch in place, with LAPACK zhegvd's storage (the runtime-n counterpart of
+ * geneig_full_n54_batched_inplace_launch): d_H receives the eigenvectors,
+ * the lower triangle of d_S the Cholesky factor.  FP64-form solvers only (a
+ * precision-form solver refines against the original H and S and refuses). */
+int geneig_solver_launch_inplace(geneig_solver_t s, void* d_H, void* d_S, double* d_W, int* d_info,
+                                 int batch_size, hipStream_t stream);
 /* NvrtcGeneigSolver::launch_chunked (nvrtc_solver.cpp:1078-1107). */
 int geneig_solver_launch_chunked(geneig_solver_t s, const void* d_H, const void* d_S,
                                  double* d_W, void* d_U, int* d_info, int total_matrices,

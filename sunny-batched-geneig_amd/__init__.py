@@ -52,6 +52,8 @@ def lib():
         "geneig_reduce_eig_n54_launch": (I, [P, P, P, P, P, P]),
         "geneig_full_n54_launch": (I, [P, P, P, P, P, P]),
         "geneig_full_n54_batched_launch": (I, [P, P, P, P, P, I, P]),
+        "geneig_full_n54_batched_inplace_launch": (I, [P, P, P, P, I, P]),
+        "geneig_solver_launch_inplace": (I, [P, P, P, P, P, I, P]),
         "geneig_finish_n54_launch": (I, [P, P, P, P, P, I, P]),
         "geneig_mixed_n54_batched_launch": (I, [P, P, P, P, P, I, I, D, P, P, P]),
         "geneig_mixed_default_max_iter": (I, []),
@@ -215,6 +217,15 @@ def full_n54_batched(H, S, W, U, info, batch=None, stream=None):
                "geneig_full_n54_batched_launch")
 
 
+def full_n54_batched_inplace(H, S, W, info, batch=None, stream=None):
+    """geneig_full_n54_batched_inplace_launch: zhegvd storage -- H receives the
+    eigenvectors, the lower triangle of S the Cholesky factor."""
+    batch = H.shape[0] if batch is None else batch
+    _check_batch(54, batch, H, S, W, H, info)
+    _check_hip(lib().geneig_full_n54_batched_inplace_launch(_ptr(H), _ptr(S), _ptr(W), _ptr(info), int(batch),
+                                                            _stream(stream)), "geneig_full_n54_batched_inplace_launch")
+
+
 def mixed_n54_batched(H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None,
                       report_fallback=True):
     """Precision form of the n = 54 batched solve (geneig_mixed_n54_batched_launch):
@@ -331,6 +342,13 @@ class GeneigSolver:
         _check_batch(self.matrix_size(), batch, H, S, W, U, info)
         _check(lib().geneig_solver_launch(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info),
                                           int(batch), _stream(stream)), "GeneigSolver.launch")
+
+    def launch_inplace(self, H, S, W, info, batch=None, stream=None):
+        """zhegvd storage: H receives the eigenvectors, the lower triangle of S the factor."""
+        batch = H.shape[0] if batch is None else batch
+        _check_batch(self.matrix_size(), batch, H, S, W, H, info)
+        _check(lib().geneig_solver_launch_inplace(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(info), int(batch),
+                                                  _stream(stream)), "GeneigSolver.launch_inplace")
 
     def launch_chunked(self, H, S, W, U, info, total, chunk_size, streams):
         _check_batch(self.matrix_size(), total, H, S, W, U, info)

@@ -55,6 +55,17 @@ __global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(
     geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, batch, smem, vecs);
 }
 
+// The Cholesky stage of the in-place solve: B factored into B (lower
+// triangle), NaN into the eigenvector output A when B is not positive definite.
+__global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol_ip_kernel(const geneig::cplx* __restrict__ B,
+                                                                            double* __restrict__ W,
+                                                                            geneig::cplx* __restrict__ A,
+                                                                            int* __restrict__ info, int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ geneig::cplx vecs[C54::VEC];
+    geneig::k1_batched<kN, 1, geneig::MODE_CHOL_IP, double>(B, B, W, A, info, nullptr, batch, batch, smem, vecs);
+}
+
 // K2 in three kernels: the QL chases of 64 pairs per wave with the rotations
 // logged (K2a), their replay into Z (K2b), and the one-pair-per-workgroup QL
 // for the rare pair whose log overflowed (K2x).
@@ -118,23 +129,31 @@ hipError_t set_lds_once(const void* kernel, unsigned bytes) {
 // Whether a mode runs the eigen stages (K2, K3) after K1.
 template <int MODE>
 constexpr bool eig_mode() {
-    return MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG || MODE == geneig::MODE_FINISH;
+    return MODE == geneig::MODE_FULL || MODE == geneig::MODE_REDUCE_EIG || MODE == geneig::MODE_FINISH ||
+           MODE == geneig::MODE_IP;
 }
+// The batched FP64 solves (MODE_FULL; MODE_IP, in place) run the Cholesky as
+// its own kernel (two waves per SIMD), then K1 from the factor.
+template <int MODE>
+constexpr bool split_chol() { return MODE == geneig::MODE_FULL || MODE == geneig::MODE_IP; }
+template <int MODE>
+constexpr int k1_mode() { return MODE == geneig::MODE_FULL ? geneig::MODE_MIXED : MODE; }
 
 // Workspace layout of a launch of `batch` pairs (the batched FP64 solve runs
 // kPlainParts concurrent parts when the batch is one chunk).
 template <int MODE>
 geneig_host::PlainLayout layout_of(int batch) {
-    return geneig_host::plain_layout(batch, W54::PAIR_BYTES, MODE == geneig::MODE_FULL ? kPlainParts : 1);
+    return geneig_host::plain_layout(batch, W54::PAIR_BYTES, split_chol<MODE>() ? kPlainParts : 1);
 }
 
 template <int MODE>
 hipError_t set_attrs() {
-    constexpr bool split_chol = (MODE == geneig::MODE_FULL);
-    auto k1 = k1_kernel<split_chol ? geneig::MODE_MIXED : MODE>;
+    auto k1 = k1_kernel<k1_mode<MODE>()>;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1), kK1Lds * kBPB));
-    if (split_chol)
-        RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
+    if (split_chol<MODE>()) {
+        auto ch = MODE == geneig::MODE_IP ? chol_ip_kernel : chol64_kernel;
+        RETURN_IF(set_lds_once(reinterpret_cast<const void*>(ch), geneig::chol64_lds_bytes<kN>()));
+    }
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3_kernel<MODE>), kK3Lds * kBPB));
     return set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds);
 }
@@ -144,8 +163,7 @@ hipError_t set_attrs() {
 template <int MODE>
 hipError_t pipeline(const geneig::cplx* a, const geneig::cplx* b, double* W, geneig::cplx* u, int* info, int off,
                     int P, unsigned char* w, hipStream_t st) {
-    constexpr bool split_chol = (MODE == geneig::MODE_FULL);
-    auto k1 = k1_kernel<split_chol ? geneig::MODE_MIXED : MODE>;
+    auto k1 = k1_kernel<k1_mode<MODE>()>;
     const size_t mat = (size_t)kN * kN;
     const unsigned g13 = (unsigned)((P + kBPB - 1) / kBPB);
     double* w_off = W ? W + (size_t)kN * off : nullptr;
@@ -154,9 +172,9 @@ hipError_t pipeline(const geneig::cplx* a, const geneig::cplx* b, double* W, gen
         hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ovf, 1);
         RETURN_IF(hipGetLastError());
     }
-    if (split_chol) {
-        hipLaunchKernelGGL(chol64_kernel, dim3((unsigned)P), dim3(64), geneig::chol64_lds_bytes<kN>(), st,
-                           b + mat * off, w_off, u + mat * off, info + off, P);
+    if (split_chol<MODE>()) {
+        hipLaunchKernelGGL(MODE == geneig::MODE_IP ? chol_ip_kernel : chol64_kernel, dim3((unsigned)P), dim3(64),
+                           geneig::chol64_lds_bytes<kN>(), st, b + mat * off, w_off, u + mat * off, info + off, P);
         RETURN_IF(hipGetLastError());
     }
     hipLaunchKernelGGL(k1, dim3(g13), dim3(64, kBPB), kK1Lds * kBPB, st, a + mat * off, b + mat * off, w_off,
@@ -276,6 +294,13 @@ int geneig_full_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleC
 unsigned int geneig_full_n54_batched_block_dim_x(void) { return 64; }
 unsigned int geneig_full_n54_batched_shared_mem_bytes(void) { return (kWorkLds + kVecLds) * kBPB; }
 
+int geneig_full_n54_batched_inplace_launch(hipDoubleComplex* d_H, hipDoubleComplex* d_S, double* d_W, int* d_info,
+                                           int batch_size, hipStream_t stream) {
+    // the eigenvectors go over H, the factor over S (U = H: K1 reads all of
+    // a pair's H before K3 writes that pair's eigenvectors)
+    return run<geneig::MODE_IP>(d_H, d_S, d_W, d_H, d_info, batch_size, stream);
+}
+
 int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex* d_M, double* d_W,
                              hipDoubleComplex* d_U, int* d_info, int batch_size,
                              hipStream_t stream) {
@@ -287,7 +312,7 @@ unsigned int geneig_finish_n54_shared_mem_bytes(void) { return (kWorkLds + kVecL
 size_t geneig_n54_workspace_bytes(int batch_size, int form) {
     if (batch_size <= 0) return 0;
     if (form == 1) return geneig_host::mixed_n54_layout(batch_size).bytes;
-    return layout_of<geneig::MODE_FULL>(batch_size).bytes;
+    return layout_of<geneig::MODE_FULL>(batch_size).bytes;   // also the in-place solve's
 }
 
 int geneig_n54_reserve(int batch_size, int form, hipStream_t stream) {

@@ -199,6 +199,8 @@ struct Module {
     hipFunction_t fn2b = nullptr;  // K2b: Z replayed from the log
     hipFunction_t fn2x = nullptr;  // K2x: one-pair QL for a pair whose log overflowed
     hipFunction_t fn3 = nullptr;   // K3: sort, Y = Q Z, back-transform
+    // in-place solve (zhegvd storage): Cholesky of B into B, K1 / K3 with L from B
+    hipFunction_t fci = nullptr, f1i = nullptr, f3i = nullptr;
     unsigned lds_per_pair = 0;     // K1 dynamic LDS bytes per pair
     unsigned lds_total = 0;        // larger of K1 / K3 dynamic LDS + static vectors, per pair
     unsigned k2_threads = 0;
@@ -248,6 +250,9 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
         HIP_TRY(hipModuleGetFunction(&m->fn2b, m->mod, "geneig_k2b"));
         HIP_TRY(hipModuleGetFunction(&m->fn2x, m->mod, "geneig_k2x"));
         HIP_TRY(hipModuleGetFunction(&m->fn3, m->mod, "geneig_k3"));
+        HIP_TRY(hipModuleGetFunction(&m->fci, m->mod, "geneig_chol_ip"));
+        HIP_TRY(hipModuleGetFunction(&m->f1i, m->mod, "geneig_k1_ip"));
+        HIP_TRY(hipModuleGetFunction(&m->f3i, m->mod, "geneig_k3_ip"));
     }
     HIP_TRY(hipModuleGetFunction(&m->fz, m->mod, "geneig_zero"));
     auto read_global = [&](const char* name, void* dst, size_t want) -> int {
@@ -264,11 +269,11 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     if (!rc) rc = read_global("geneig_k2a_lds_bytes", &m->k2a_lds, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_k3_lds_bytes", &m->k3_lds, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
+    if (!rc) rc = read_global("geneig_chol64_lds_bytes", &m->chol64_lds, sizeof(unsigned));
     if (variant == 1) {
         if (!rc) rc = read_global("geneig_f_lds_bytes_per_pair", &m->f_lds_per_pair, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2_threads", &m->f_k2_threads, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2a_lds_bytes", &m->f_k2a_lds, sizeof(unsigned));
-        if (!rc) rc = read_global("geneig_chol64_lds_bytes", &m->chol64_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k3_lds_bytes", &m->f_k3_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_ws_bytes_per_pair", &m->f_ws_pair_bytes, sizeof(unsigned long long));
         if (!rc) rc = read_global("geneig_refine_lds_bytes", &m->refine_lds, sizeof(unsigned));
@@ -479,8 +484,10 @@ struct geneig_solver {
     }
 
     // K1 -> K2 -> K3 over the batch, chunked to bound the per-stream workspace.
+    // inplace: LAPACK zhegvd storage (U is H: the eigenvectors overwrite H,
+    // the Cholesky factor S), with the Cholesky as its own kernel.
     int launch_raw(const Module& m, int b, const void* H, const void* S, double* W, void* U, int* info,
-                   int batch, hipStream_t stream) {
+                   int batch, hipStream_t stream, bool inplace = false) {
         if (batch <= 0) return GENEIG_E_OK;
         const geneig_host::PlainLayout L = plain_layout_of(m, batch);
         void* ws = nullptr;
@@ -498,9 +505,14 @@ struct geneig_solver {
             int one = 1;
             void* az[] = {&ovf, &one};
             HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, az, nullptr));
+            if (inplace) {
+                void* ac[] = {&sp, &w, &u, &inf, &P};
+                HIP_TRY(hipModuleLaunchKernel(m.fci, (unsigned)P, 1, 1, 64, 1, 1, m.chol64_lds, stream, ac, nullptr));
+            }
             void* a1[] = {&h, &sp, &w, &u, &inf, &ws, &P};
             const unsigned g13 = (unsigned)((P + b - 1) / b);
-            HIP_TRY(hipModuleLaunchKernel(m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(inplace ? m.f1i : m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1,
+                                          nullptr));
             int cap = geneig_host::ql_log_entries();
             void* a2a[] = {&inf, &ws, &P, &cap, &ovf};
             HIP_TRY(hipModuleLaunchKernel(m.fn2a, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a,
@@ -511,8 +523,8 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)std::min(P, kListGridK2x), 1, 1, m.k2_threads, 1, 1, 0,
                                           stream, a2x, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.fn3, g13, 1, 1, 64, (unsigned)b, 1, m.k3_lds * (unsigned)b, stream, a3,
-                                          nullptr));
+            HIP_TRY(hipModuleLaunchKernel(inplace ? m.f3i : m.fn3, g13, 1, 1, 64, (unsigned)b, 1,
+                                          m.k3_lds * (unsigned)b, stream, a3, nullptr));
         }
         return GENEIG_E_OK;
     }
@@ -617,9 +629,13 @@ struct geneig_solver {
         const int bytes = (int)(m.lds_per_pair * (unsigned)b);
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn2a), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)m.k2a_lds);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fn3), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(m.k3_lds * (unsigned)b));
+        for (hipFunction_t f : {m.fn, m.f1i})
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        for (hipFunction_t f : {m.fn3, m.f3i})
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(f), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(m.k3_lds * (unsigned)b));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(m.fci), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)m.chol64_lds);
     }
 
     int autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands, TuneWinner& win);
@@ -844,6 +860,15 @@ int geneig_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, do
     if (s->variant == 1)
         return s->launch_mixed(*s->module, d_H, d_S, d_W, d_U, d_info, batch_size, 0, 0.0, nullptr, nullptr, stream);
     return s->launch_raw(*s->module, s->bpb, d_H, d_S, d_W, d_U, d_info, batch_size, stream);
+}
+
+int geneig_solver_launch_inplace(geneig_solver_t s, void* d_H, void* d_S, double* d_W, int* d_info, int batch_size,
+                                 hipStream_t stream) {
+    if (!s) return fail(GENEIG_E_ARG, "null solver");
+    if (s->variant == 1)
+        return fail(GENEIG_E_ARG, "launch_inplace: the precision form refines against the original H and S, "
+                                  "which an in-place solve overwrites; use a solver of the FP64 form");
+    return s->launch_raw(*s->module, s->bpb, d_H, d_S, d_W, d_H, d_info, batch_size, stream, true);
 }
 
 int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int cache_mode) {

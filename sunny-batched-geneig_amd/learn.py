@@ -179,17 +179,21 @@ class WarmStart:
         self.adam(lr, stream=stream)
         return loss
 
-    def train(self, X, Cset, epochs, batch=256, lr=1e-3, lr_final=1e-4, seed=0, log=None, spike=1.5):
+    def train(self, X, Cset, epochs, batch=256, lr=1e-3, lr_final=1e-4, seed=0, log=None, spike=1.5,
+              nan_spike=0.01):
         """Adam over shuffled mini-batches, learning rate decayed geometrically
-        from lr to lr_final.  Returns the mean loss per epoch.
+        from lr to lr_final.  Returns the mean loss per epoch over the samples
+        whose loss is finite (``nan_counts`` holds the others per epoch).
 
         Loss spikes: a network output close to rank deficiency makes the
-        Cholesky-QR gradient huge, and one such step can throw the weights far
-        off (the loss then stays high).  The weights and Adam moments of the
-        best epoch so far are kept; an epoch whose mean loss exceeds `spike`
-        times the best returns to them with the learning rate halved from then
-        on, and training ends on the best epoch's weights (``train_best`` is
-        its mean loss; ``rollbacks`` counts the returns)."""
+        Cholesky-QR gradient huge (and an exactly rank-deficient one a NaN
+        loss), and one such step can throw the weights far off (the loss then
+        stays high).  The weights and Adam moments of the best epoch so far are
+        kept; an epoch whose mean loss exceeds `spike` times the best, or whose
+        fraction of NaN losses exceeds `nan_spike`, returns to them with the
+        learning rate halved from then on and is never kept as the best; NaN
+        samples never enter a mean.  Training ends on the best epoch's weights
+        (``train_best`` is its mean loss; ``rollbacks`` counts the returns)."""
         torch = _torch()
         P = X.shape[0]
         gen = torch.Generator(device="cpu").manual_seed(seed)
@@ -199,30 +203,37 @@ class WarmStart:
         best = None
         factor = 1.0
         self.rollbacks = 0
+        self.nan_counts = []
         for ep in range(epochs):
             perm = torch.randperm(P, generator=gen).to(torch.int32).to(self.device)
             tot = torch.zeros((), dtype=torch.float64, device=self.device)
+            nans = torch.zeros((), dtype=torch.int64, device=self.device)
             cnt = 0
             for off in range(0, P, batch):
                 idx = perm[off:off + batch].contiguous()
                 rate = factor * lr * (lr_final / lr) ** (k / max(1, steps - 1))
-                l = self.train_step(X.index_select(0, idx.long()).contiguous(), Cset, idx, rate)
-                tot += torch.nan_to_num(l.double(), nan=0.0).sum()
+                l = self.train_step(X.index_select(0, idx.long()).contiguous(), Cset, idx, rate).double()
+                bad = torch.isnan(l)
+                tot += torch.where(bad, torch.zeros_like(l), l).sum()
+                nans += bad.sum()
                 cnt += idx.numel()
                 k += 1
-            hist.append(float((tot / cnt).item()))
+            n_nan = int(nans.item())
+            self.nan_counts.append(n_nan)
+            hist.append(float((tot / (cnt - n_nan)).item()) if cnt > n_nan else float("nan"))
+            degenerate = n_nan > nan_spike * cnt
             if log:
-                log("epoch %d: mean loss %.5f" % (ep, hist[-1]))
-            if best is None or hist[-1] < best[0]:
+                log("epoch %d: mean loss %.5f (%d NaN of %d)" % (ep, hist[-1], n_nan, cnt))
+            if not degenerate and (best is None or hist[-1] < best[0]):
                 best = (hist[-1], self.flat.clone(), self.m.clone(), self.v.clone(), self.step)
-            elif hist[-1] > spike * best[0]:
+            elif best is not None and (degenerate or hist[-1] > spike * best[0]):
                 self._restore(best)
                 factor *= 0.5
                 self.rollbacks += 1
                 if log:
-                    log("epoch %d: loss spike (best %.5f): back to the best weights, learning rate x%.3g"
-                        % (ep, best[0], factor))
-        if best is not None and hist[-1] > best[0]:
+                    log("epoch %d: loss spike (best %.5f, %d NaN): back to the best weights, learning rate x%.3g"
+                        % (ep, best[0], n_nan, factor))
+        if best is not None and not (hist[-1] <= best[0] and self.nan_counts[-1] <= nan_spike * P):
             self._restore(best)
         self.train_best = best[0] if best is not None else float("nan")
         return hist

@@ -171,3 +171,11 @@ def test_noalloc_refuses_oversized_batch_before_device_work(geneig):
     assert rc != 0
     assert "split the batch" in geneig.last_error() and str(mx) in geneig.last_error()
     assert L.geneig_full_n54_batched_launch_noalloc(None, None, None, None, None, 0, None) == 0
+
+
+def test_rtc_refuses_non_cdna_target(geneig):
+    """The precision-form module builds gfx9-format buffer descriptors and uses
+    CDNA matrix-core instructions: compiling it for another target fails
+    with a message naming the requirement instead of producing wrong code."""
+    with pytest.raises(geneig.GeneigError, match="gfx9"):
+        geneig.rtc_compile(54, 1, variant=1, arch="gfx1100")

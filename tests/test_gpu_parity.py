@@ -380,3 +380,53 @@ def test_workspace_reserve_and_release(geneig, oracle, device):
     Ah, Bh = to_host(A[:32]), to_host(B[:32])
     Wo, Xo, _ = oracle.zhegv(Ah, Bh)
     check_pairs(oracle, Ah, Bh, to_host(outs[1][0][:32]), to_host(outs[1][1][:32]), Wo, Xo, 54, "precision after reserve")
+
+
+def test_inplace_matches_out_of_place(geneig, oracle, device):
+    """The in-place solve (zhegvd storage) leaves in H exactly the eigenvectors
+    the out-of-place launch writes to U, the same W and info, and in the lower
+    triangle of S a factor with L L^H = S; a pair whose B is not positive
+    definite gets info = n + j and NaN in its H and W.  The static n = 54
+    entry (20,000 pairs: concurrent parts) and a runtime n = 40 solver."""
+    import torch
+
+    total = 20000
+    A, B = geneig.gen_spinwave(27, total, seed=17)
+    B[5, 3, 3] = -1.0                        # not positive definite: leading minor 4
+    W, U, info = geneig.empty_outputs(54, total, device)
+    geneig.full_n54_batched(A, B, W, U, info)
+    H, S = A.clone(), B.clone()
+    W2 = torch.empty_like(W)
+    i2 = torch.empty_like(info)
+    geneig.full_n54_batched_inplace(H, S, W2, i2)
+    torch.cuda.synchronize()
+    assert int(i2[5]) == 54 + 4 and int(info[5]) == 54 + 4
+    assert bool(torch.isnan(H[5]).all()) and bool(torch.isnan(W2[5]).all())
+    ok = torch.ones(total, dtype=torch.bool, device=device)
+    ok[5] = False
+    assert torch.equal(info, i2)
+    assert torch.equal(W[ok], W2[ok]) and torch.equal(U[ok], H[ok])
+    # S holds L (storage [b, j, i] = element (i, j): the lower triangle is L^T's upper)
+    for p in (0, 17, total - 1):
+        L = np.tril(to_host(S[p]).T)
+        Bp = to_host(B[p]).T
+        assert np.abs(L @ L.conj().T - Bp).max() <= 1e-13 * np.abs(Bp).max(), p
+    Ah, Bh = to_host(A[:32]), to_host(B[:32])
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    sel = [p for p in range(32) if p != 5]
+    check_pairs(oracle, Ah[sel], Bh[sel], to_host(W2[:32])[sel], to_host(H[:32])[sel], Wo[sel], Xo[sel], 54,
+                "in place n=54")
+
+    s40 = geneig.GeneigSolver(40)
+    Ag, Bg = geneig.gen_generic(40, 300, seed=18)
+    Wg, Ug, ig = geneig.empty_outputs(40, 300, device)
+    s40.launch(Ag, Bg, Wg, Ug, ig)
+    Hg, Sg = Ag.clone(), Bg.clone()
+    Wg2, ig2 = torch.empty_like(Wg), torch.empty_like(ig)
+    s40.launch_inplace(Hg, Sg, Wg2, ig2)
+    torch.cuda.synchronize()
+    assert int(ig.abs().max()) == 0 and torch.equal(ig, ig2)
+    assert torch.equal(Wg, Wg2) and torch.equal(Ug, Hg)
+    sm = geneig.GeneigMixedSolver(40)
+    with pytest.raises(geneig.GeneigError, match="precision form"):
+        sm.launch_inplace(Hg, Sg, Wg2, ig2)
