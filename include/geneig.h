@@ -95,6 +95,29 @@ unsigned int geneig_full_n54_batched_shared_mem_bytes(void);  /* geneig_fixed.cu
 int geneig_full_n54_batched_inplace_launch(hipDoubleComplex* d_H, hipDoubleComplex* d_S, double* d_W,
                                            int* d_info, int batch_size, hipStream_t stream);
 
+/* Spin-wave pairs built inside the solve.  The pair of q = d_q[3b .. 3b+2] is
+ * (eta, H(q) + d_diag[b] I) with eta = diag(I_27, -I_27) and
+ *   H(q) = [[Aq, Bq], [conj Bq, conj Aq]]:  bond k joins site i to site
+ *   t = (i + d_shift[k]) mod 27 with coupling J = d_J[27 k + i] and offset
+ *   r = d_r[3k .. 3k+2]; with theta = 2 pi q.r, it adds J/2 e^{i theta} to
+ *   Aq(i, t) and Bq(i, t), J/2 e^{-i theta} to Aq(t, i), J/2 e^{i theta} to
+ *   Bq(t, i), and J to Aq(i, i)  (d_diag may be NULL: no shift).
+ * The FP64 Cholesky stage builds each column of H(q) in registers from these
+ * tables and K1 builds eta, so neither matrix of the pair is ever stored in
+ * or read from device memory; outputs W, U, info as
+ * geneig_full_n54_batched_launch.  The spin-wave consumer of the reference
+ * (benchmarks/sw08_baseline.jl:163-174) builds H(q) on the host and passes it
+ * whole; this entry has no reference counterpart. */
+int geneig_spinwave_n54_launch(int nbonds, const int* d_shift, const int* d_r, const double* d_J,
+                               const double* d_q, const double* d_diag, double* d_W, hipDoubleComplex* d_U,
+                               int* d_info, int batch_size, hipStream_t stream);
+/* The H(q) + diag I the launch above builds, written to d_H (batch x 54 x 54,
+ * column-major): for checking the construction, or for callers that need the
+ * matrices themselves. */
+int geneig_spinwave_n54_build(int nbonds, const int* d_shift, const int* d_r, const double* d_J,
+                              const double* d_q, const double* d_diag, hipDoubleComplex* d_H, int batch_size,
+                              hipStream_t stream);
+
 /* Batched finish from a given factor L (lower, zero upper) and reduced M:
  * heev(M) then U = L^{-H} V.
  * Replaces geneig_finish_n54_launch, src/geneig_fixed.cu:569-587. */

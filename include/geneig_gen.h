@@ -43,6 +43,13 @@ int geneig_spinwave_params_launch(uint64_t seed, int g0, int g1, int g2, uint64_
                                   double jscale, float* d_params, hipStream_t stream);
 int geneig_fill_identity_launch(int n, int batch, void* d_B, hipStream_t stream);
 
+/* The spin-wave family's model as the tables geneig_spinwave_n54_launch
+ * (geneig.h) takes: d_shift[b] and d_r[3b .. 3b+2] of each of the nbonds
+ * bonds, and d_J[b nsub + i] = jscale (0.5 + U) the coupling of bond b at
+ * site i (jscale >= 0; the draws of geneig_gen_spinwave_h0_launch). */
+int geneig_spinwave_model_launch(int nsub, int nbonds, uint64_t seed, double jscale, int* d_shift, int* d_r,
+                                 double* d_J, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

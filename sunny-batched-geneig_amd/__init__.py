@@ -115,6 +115,9 @@ def lib():
         "geneig_gen_spinwave_h0_launch": (I, [I, I, U64, I, I, I, U64, I, D, P, P, P]),
         "geneig_spinwave_shift_launch": (I, [I, I, U64, U64, P, P, P]),
         "geneig_fill_identity_launch": (I, [I, I, P, P]),
+        "geneig_spinwave_model_launch": (I, [I, I, U64, D, P, P, P, P]),
+        "geneig_spinwave_n54_launch": (I, [I, P, P, P, P, P, P, P, P, I, P]),
+        "geneig_spinwave_n54_build": (I, [I, P, P, P, P, P, P, I, P]),
     }
     for name in ("chol_n54", "geneig_reduce_n54", "geneig_reduce_eig_n54", "geneig_full_n54",
                  "geneig_full_n54_batched", "geneig_finish_n54"):
@@ -572,6 +575,57 @@ def gen_spinwave(nsub, batch, seed, grid=(64, 32, 32), first_q=0, nbonds=12, jsc
         raise GeneigError("spin-wave generator: lambda_min solve reported info != 0")
     del Hc, Ic, Wc, Uc, infoc
     return A, B
+
+
+def spinwave_model(nsub, seed, nbonds=12, jscale=1.0, device="cuda", stream=None):
+    """The spin-wave family's bond tables (shift, r, J) on the device, as
+    geneig_spinwave_n54_launch takes them (the draws of gen_spinwave)."""
+    torch = _torch()
+    shift = torch.empty(nbonds, dtype=torch.int32, device=device)
+    r = torch.empty((nbonds, 3), dtype=torch.int32, device=device)
+    J = torch.empty((nbonds, nsub), dtype=torch.float64, device=device)
+    _check_hip(lib().geneig_spinwave_model_launch(int(nsub), int(nbonds), int(seed), float(jscale), _ptr(shift),
+                                                  _ptr(r), _ptr(J), _stream(stream)), "geneig_spinwave_model_launch")
+    return shift, r, J
+
+
+def spinwave_q(batch, grid=(64, 32, 32), first_q=0, device="cuda"):
+    """q of grid points first_q .. first_q + batch - 1, index (i g1 + j) g2 + k,
+    q = (i/g0, j/g1, k/g2) (exact in float64), as a (batch, 3) device tensor."""
+    import numpy as np
+
+    torch = _torch()
+    g0, g1, g2 = grid
+    qi = np.arange(first_q, first_q + batch, dtype=np.int64)
+    q = np.stack([(qi // (g2 * g1)) % g0 / g0, (qi // g2) % g1 / g1, qi % g2 / g2], axis=1)
+    return torch.from_numpy(np.ascontiguousarray(q, dtype=np.float64)).to(device)
+
+
+def spinwave_n54(model, q, diag, W, U, info, batch=None, stream=None):
+    """geneig_spinwave_n54_launch: solve (eta, H(q) + diag I) with H(q) built inside
+    the solve from the model tables (shift, r, J); diag may be None."""
+    torch = _torch()
+    shift, r, J = model
+    batch = q.shape[0] if batch is None else batch
+    _require(q, torch.float64, "q")
+    if diag is not None:
+        _require(diag, torch.float64, "diag")
+    _check_batch(54, batch, U, U, W, U, info)
+    _check_hip(lib().geneig_spinwave_n54_launch(int(shift.numel()), _ptr(shift), _ptr(r), _ptr(J), _ptr(q),
+                                                _ptr(diag), _ptr(W), _ptr(U), _ptr(info), int(batch), _stream(stream)),
+               "geneig_spinwave_n54_launch")
+
+
+def spinwave_n54_build(model, q, diag=None, stream=None):
+    """The H(q) + diag I matrices the fused solve builds, as a (batch, 54, 54) tensor."""
+    torch = _torch()
+    shift, r, J = model
+    batch = q.shape[0]
+    H = torch.empty((batch, 54, 54), dtype=torch.complex128, device=q.device)
+    _check_hip(lib().geneig_spinwave_n54_build(int(shift.numel()), _ptr(shift), _ptr(r), _ptr(J), _ptr(q),
+                                               _ptr(diag), _ptr(H), int(batch), _stream(stream)),
+               "geneig_spinwave_n54_build")
+    return H
 
 
 from . import learn  # noqa: E402  (learned warm start; needs the names above)

@@ -66,6 +66,54 @@ __global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol_ip_kernel
     geneig::k1_batched<kN, 1, geneig::MODE_CHOL_IP, double>(B, B, W, A, info, nullptr, batch, batch, smem, vecs);
 }
 
+// Fused spin-wave solve: the Cholesky stage builds its pair's H(q) + diag I
+// column per lane from the bond table (spinwave_column) instead of reading B,
+// and stashes L in U; A = eta is built inside K1 (MODE_SW).
+struct SpinwaveModel {
+    int nbonds;
+    const int* shift;      // nbonds
+    const int* r;          // nbonds x 3
+    const double* J;       // nbonds x 27
+    const double* q;       // batch x 3
+    const double* diag;    // batch, or null
+};
+
+__global__ void __launch_bounds__(64, 1) chol_sw_kernel(SpinwaveModel m, double* __restrict__ W,
+                                                                            geneig::cplx* __restrict__ U,
+                                                                            int* __restrict__ info, int off,
+                                                                            int batch) {
+    __shared__ geneig::cplx colv[C54::VEC];
+    const int lane = threadIdx.x;
+    const long long p = (long long)off + blockIdx.x;
+    if (blockIdx.x >= (unsigned)batch) return;
+    const size_t mat = (size_t)kN * kN;
+    geneig::cplx a[kN];
+    const double d =
+        geneig::spinwave_column<kN>(a, lane, m.nbonds, m.shift, m.r, m.J, m.q + 3 * p, m.diag ? m.diag[p] : 0.0);
+    const int bad = geneig::chol_core<kN, 4, false>(a, d, U + mat * p, colv, lane);
+    if (bad) {
+        if (lane == 0) info[p] = kN + bad;
+        geneig::write_nan<kN>(W + (size_t)kN * p, U + mat * p, lane, true);
+        return;
+    }
+    if (lane == 0) info[p] = 0;
+}
+
+// H(q) + diag I as the fused solve builds it, written out (for checking the
+// builder against an independent construction).
+__global__ void __launch_bounds__(64) sw_build_kernel(SpinwaveModel m, geneig::cplx* __restrict__ H, int batch) {
+    const int lane = threadIdx.x;
+    const long long p = blockIdx.x;
+    if (p >= batch) return;
+    geneig::cplx a[kN];
+    geneig::spinwave_column<kN>(a, lane, m.nbonds, m.shift, m.r, m.J, m.q + 3 * p, m.diag ? m.diag[p] : 0.0);
+    if (lane < kN) {
+        geneig::cplx* col = H + (size_t)kN * kN * p + (size_t)kN * lane;
+#pragma unroll
+        for (int i = 0; i < kN; ++i) col[i] = a[i];
+    }
+}
+
 // K2 in three kernels: the QL chases of 64 pairs per wave with the rotations
 // logged (K2a), their replay into Z (K2b), and the one-pair-per-workgroup QL
 // for the rare pair whose log overflowed (K2x).
@@ -232,6 +280,61 @@ hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int 
     return hipSuccess;
 }
 
+// The fused spin-wave solve: per part, the Cholesky stage from the model,
+// then K1 (A = eta) .. K3 as the batched FP64 solve.
+hipError_t pipeline_sw(const SpinwaveModel& m, double* W, geneig::cplx* u, int* info, int off, int P,
+                       unsigned char* w, hipStream_t st) {
+    auto k1 = k1_kernel<geneig::MODE_SW>;
+    const size_t mat = (size_t)kN * kN;
+    int* ovf = geneig_host::region_ctl(w, P, W54::PAIR_BYTES);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ovf, 1);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(chol_sw_kernel, dim3((unsigned)P), dim3(64), 0, st, m, W, u, info, off, P);
+    RETURN_IF(hipGetLastError());
+    double* w_off = W + (size_t)kN * off;
+    geneig::cplx* u_off = u + mat * off;
+    hipLaunchKernelGGL(k1, dim3((unsigned)P), dim3(64, kBPB), kK1Lds * kBPB, st, u_off, u_off, w_off, u_off,
+                       info + off, w, P);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, st, info + off, w, P,
+                       geneig_host::ql_log_entries(), ovf);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)std::min(P, geneig::kListGridK2x)), dim3(64 * C54::R), 0, st,
+                       info + off, w, P, (const int*)ovf);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k3_kernel<geneig::MODE_FULL>, dim3((unsigned)P), dim3(64, kBPB), kK3Lds * kBPB, st, u_off,
+                       w_off, u_off, info + off, w, P);
+    return hipGetLastError();
+}
+
+hipError_t run_sw(const SpinwaveModel& m, double* W, geneig::cplx* U, int* info, int batch, hipStream_t stream) {
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1_kernel<geneig::MODE_SW>), kK1Lds * kBPB));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3_kernel<geneig::MODE_FULL>), kK3Lds * kBPB));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
+    if (batch <= 0) return hipSuccess;
+    const geneig_host::PlainLayout L = layout_of<geneig::MODE_FULL>(batch);
+    void* ws = nullptr;
+    RETURN_IF(geneig_host::workspace(stream, L.bytes, &ws));
+    unsigned char* base = static_cast<unsigned char*>(ws);
+    if (L.parts > 1) {
+        hipStream_t subs[geneig_host::kMaxParts];
+        RETURN_IF(geneig_host::fork_streams(stream, L.parts, subs));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < L.parts && e == hipSuccess; ++i) {
+            const int off = (int)((long long)batch * i / L.parts);
+            const int P = (int)((long long)batch * (i + 1) / L.parts) - off;
+            e = pipeline_sw(m, W, U, info, off, P, base + L.region * i, subs[i]);
+        }
+        const hipError_t j = geneig_host::join_streams(stream, L.parts, subs);
+        return e != hipSuccess ? e : j;
+    }
+    for (int off = 0; off < batch; off += L.chunk)
+        RETURN_IF(pipeline_sw(m, W, U, info, off, std::min(L.chunk, batch - off), base, stream));
+    return hipSuccess;
+}
+
 }  // namespace
 
 namespace geneig_host {
@@ -308,6 +411,25 @@ int geneig_finish_n54_launch(const hipDoubleComplex* d_L, const hipDoubleComplex
 }
 unsigned int geneig_finish_n54_block_dim_x(void) { return 64; }
 unsigned int geneig_finish_n54_shared_mem_bytes(void) { return (kWorkLds + kVecLds) * kBPB; }
+
+int geneig_spinwave_n54_launch(int nbonds, const int* d_shift, const int* d_r, const double* d_J, const double* d_q,
+                               const double* d_diag, double* d_W, hipDoubleComplex* d_U, int* d_info, int batch_size,
+                               hipStream_t stream) {
+    if (batch_size <= 0) return hipSuccess;
+    if (nbonds < 0 || (nbonds > 0 && (!d_shift || !d_r || !d_J)) || !d_q) return hipErrorInvalidValue;
+    const SpinwaveModel m{nbonds, d_shift, d_r, d_J, d_q, d_diag};
+    return run_sw(m, d_W, reinterpret_cast<geneig::cplx*>(d_U), d_info, batch_size, stream);
+}
+
+int geneig_spinwave_n54_build(int nbonds, const int* d_shift, const int* d_r, const double* d_J, const double* d_q,
+                              const double* d_diag, hipDoubleComplex* d_H, int batch_size, hipStream_t stream) {
+    if (batch_size <= 0) return hipSuccess;
+    if (nbonds < 0 || (nbonds > 0 && (!d_shift || !d_r || !d_J)) || !d_q || !d_H) return hipErrorInvalidValue;
+    const SpinwaveModel m{nbonds, d_shift, d_r, d_J, d_q, d_diag};
+    hipLaunchKernelGGL(sw_build_kernel, dim3((unsigned)batch_size), dim3(64), 0, stream, m,
+                       reinterpret_cast<geneig::cplx*>(d_H), batch_size);
+    return hipGetLastError();
+}
 
 size_t geneig_n54_workspace_bytes(int batch_size, int form) {
     if (batch_size <= 0) return 0;

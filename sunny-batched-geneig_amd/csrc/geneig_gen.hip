@@ -179,6 +179,22 @@ __global__ void spinwave_params_kernel(uint64_t seed, int g0, int g1, int g2, ui
     out[5 * b + 4] = (float)pow(10.0, -3.0 + 2.0 * u01(seed, 7, qi, 0));
 }
 
+// The family's model as tables: bond b's site shift and offset r_b, and its
+// coupling at site i, J[b nsub + i] = jscale (0.5 + U) (the draws of
+// gen_spinwave_kernel).
+__global__ void spinwave_model_kernel(int nsub, int nbonds, uint64_t seed, double jscale, int* __restrict__ shift,
+                                      int* __restrict__ r, double* __restrict__ J) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nbonds) {
+        const Bond bo = bond(seed, nsub, t);
+        shift[t] = bo.shift;
+        r[3 * t] = bo.r0;
+        r[3 * t + 1] = bo.r1;
+        r[3 * t + 2] = bo.r2;
+    }
+    if (t < nbonds * nsub) J[t] = jscale * (0.5 + u01(seed, 6, (uint64_t)(t / nsub), (uint32_t)(t % nsub)));
+}
+
 __global__ void identity_kernel(int n, double2* __restrict__ B, long long total) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -235,6 +251,15 @@ int geneig_spinwave_params_launch(uint64_t seed, int g0, int g1, int g2, uint64_
     hipLaunchKernelGGL(spinwave_params_kernel, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, stream, seed, g0,
                        g1, g2, first_q, jscale, d_params, batch);
     return hipGetLastError();
+}
+
+int geneig_spinwave_model_launch(int nsub, int nbonds, uint64_t seed, double jscale, int* d_shift, int* d_r,
+                                 double* d_J, hipStream_t stream) {
+    if (nsub < 2 || nbonds < 1 || !(jscale >= 0.0) || !d_shift || !d_r || !d_J) return (int)hipErrorInvalidValue;
+    const int total = nbonds * nsub;
+    hipLaunchKernelGGL(spinwave_model_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, nsub,
+                       nbonds, seed, jscale, d_shift, d_r, d_J);
+    return (int)hipGetLastError();
 }
 
 int geneig_fill_identity_launch(int n, int batch, void* d_B, hipStream_t stream) {
