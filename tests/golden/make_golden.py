@@ -9,6 +9,11 @@ LAPACK for the own_* files):  python tests/golden/make_golden.py
   kept as data so the parity tests can run where the reference is absent.
 * own_*.bin -- this project's synthetic families (the oracle's counter-based
   generators) with eigenpairs from LAPACK zhegv (MKL or scipy's OpenBLAS).
+* ref_wu_n{N}_{mode}.npz (``--reference-wu``) -- W and U of all 8 pairs of each
+  reference fixture (arrays W: 8 x N, U: 8 x N x N in the GEIG storage).  The
+  inputs are not stored: sunny-batched-geneig_amd/geig.py regenerates them
+  bit for bit from the seed (checked against the reference's files where they
+  are present), so the parity tests cover every pair at a fraction of the size.
 """
 import os
 import sys
@@ -25,7 +30,21 @@ REF = "/root/reference/tests/data"
 KEEP = {16: 2, 32: 2, 48: 1, 54: 2, 64: 1, 96: 1}
 
 
+def write_reference_wu():
+    if not os.path.isdir(REF):
+        raise SystemExit("reference fixtures not present")
+    for n in KEEP:
+        for mode in ("random", "nearid"):
+            fx = O.geig_read(os.path.join(REF, "n%d_b8_s42_%s.bin" % (n, mode)))
+            dst = os.path.join(HERE, "ref_wu_n%d_%s.npz" % (n, mode))
+            np.savez(dst, W=np.ascontiguousarray(fx["W"]), U=np.ascontiguousarray(fx["U"]))
+            print("wrote", dst, fx["W"].shape)
+
+
 def main():
+    if "--reference-wu" in sys.argv[1:]:
+        write_reference_wu()
+        return
     if os.path.isdir(REF):
         for n, keep in KEEP.items():
             for mode in ("random", "nearid"):

@@ -103,3 +103,32 @@ def test_dump_report(G):
     assert lines[1] == "magic:   0x47454947" and lines[3] == "N:       54" and lines[5] == "seed:    42"
     assert all("sorted_ascending=yes" in l for l in lines if l.startswith("batch "))
     assert lines[-1].startswith("max Hermiticity floor on H:      0.000e+00")
+
+
+REF_DATA = "/root/reference/tests/data"
+WU_FILES = sorted(glob.glob(os.path.join(GOLDEN, "ref_wu_n*_*.npz")))
+
+
+@pytest.mark.parametrize("path", WU_FILES, ids=lambda p: os.path.basename(p))
+def test_reference_wu_fixtures(G, path):
+    """ref_wu_*.npz hold W and U of all 8 pairs of a reference fixture; with the
+    inputs regenerated from the seed they replace the whole file.  Where the
+    reference's files are present, the regenerated H, S of all 8 pairs equal
+    the reference's bit for bit and W, U equal its stored outputs."""
+    d = np.load(path)
+    W, U = d["W"], d["U"]
+    name = os.path.basename(path)[len("ref_wu_"):-len(".npz")]        # n{N}_{mode}
+    n = int(name.split("_")[0][1:])
+    assert W.shape == (8, n) and U.shape == (8, n, n) and W.dtype == np.float64 and U.dtype == np.complex128
+    assert (np.diff(W, axis=1) >= 0).all()                              # ascending, as zhegvd returns them
+    src = os.path.join(REF_DATA, "%s_b8_s42_%s.bin" % (name.split("_")[0], name.split("_")[1]))
+    if not os.path.exists(src):
+        pytest.skip("the reference's fixtures are not on this host")
+    ref = G.read(src)
+    mode = "random" if name.endswith("random") else "near-identity"
+    H, S = G.reference_pairs(n, 8, 42, mode)
+    assert H.tobytes() == ref["H"].tobytes() and S.tobytes() == ref["S"].tobytes()
+    # geig.py returns matrices in index order (i, j); the npz keeps the
+    # column-major storage order of the file and the device, [b, j, i]
+    assert W.tobytes() == ref["W"].tobytes()
+    assert U.tobytes() == np.ascontiguousarray(np.swapaxes(ref["U"], -1, -2)).tobytes()

@@ -82,6 +82,33 @@ def test_mixed_not_positive_definite(geneig, oracle, device):
     check_pairs(oracle, A[keep], B[keep], W[keep], X[keep], Wo[keep], Xo[keep], 54, "mixed non-PD neighbours")
 
 
+def compare_to_fp64_path(oracle, A, B, n, cond, W, X, Wp, Xp, tag):
+    """The precision form's outputs (W, X) and the FP64 path's (Wp, Xp) on the
+    same pairs against the oracle: every metric meets the FP64 bar or stays
+    within twice the FP64 path's figure; at cond(B) >= 1e6 the eigenvalues are
+    checked against long-double Rayleigh quotients of the oracle's
+    eigenvectors instead (the oracle's own FP64 eigenvalues are off by
+    ~ eps cond(B) there), with the first-order bound eps cond(B)."""
+    from test_gpu_parity import EPS, TOL_EIG, TOL_PHASE
+
+    Wo, Xo, io = oracle.zhegv(A, B)
+    assert (io == 0).all()
+    mp = oracle.compare(A, B, Wp, Xp, Wo, Xo)
+    mm = oracle.compare(A, B, W, X, Wo, Xo)
+    bars = {"eig_rel_scaled": TOL_EIG, "phase_err": TOL_PHASE, "resid": 50 * n * EPS, "orth": 50 * n * EPS}
+    for k, bar in bars.items():
+        print("  %s %-15s mixed %.2e  fp64 path %.2e  bar %.1e" % (tag, k, mm[k], mp[k], bar), flush=True)
+        if k == "eig_rel_scaled" and cond >= 1e6:
+            continue
+        assert mm[k] <= max(bar, 2.0 * mp[k]), (tag, k)
+    if cond >= 1e6:
+        R = oracle.rayleigh_ld(A, B, Xo)
+        em, ep, eo = (oracle.eig_err_scaled(w, R) for w in (W, Wp, Wo))
+        print("  %s eig vs long-double Rayleigh quotients: mixed %.2e  fp64 path %.2e  oracle %.2e  bar %.1e"
+              % (tag, em, ep, eo, EPS * cond), flush=True)
+        assert em <= EPS * cond and ep <= EPS * cond, tag
+
+
 @pytest.mark.parametrize("cond", [1e2, 1e4, 1e6, 1e8])
 def test_mixed_condition_sweep(geneig, oracle, device, cond):
     """cond(B) from 1e2 to 1e8: the refinement converges or the pair is re-solved
@@ -96,32 +123,51 @@ def test_mixed_condition_sweep(geneig, oracle, device, cond):
     relative perturbation of B."""
     import torch
 
-    from test_gpu_parity import EPS, TOL_EIG, TOL_PHASE
-
     A, B = oracle.gen_generic_cond(54, 64, seed=3, cond=cond)
     W, X, info, iters, fb = mixed(geneig, A, B, device)
     assert (info == 0).all()
-    Wo, Xo, io = oracle.zhegv(A, B)
     W2, U2, i2 = geneig.empty_outputs(54, 64, device)
     geneig.full_n54_batched(to_dev(A, device), to_dev(B, device), W2, U2, i2)
     torch.cuda.synchronize()
-    mp = oracle.compare(A, B, to_host(W2), to_host(U2), Wo, Xo)
-    mm = oracle.compare(A, B, W, X, Wo, Xo)
     print("cond %.0e: fallback %d of 64, refinement steps %s" % (cond, fb, hist(iters)), flush=True)
-    bars = {"eig_rel_scaled": TOL_EIG, "phase_err": TOL_PHASE, "resid": 50 * 54 * EPS, "orth": 50 * 54 * EPS}
-    for k, bar in bars.items():
-        print("  %-15s mixed %.2e  fp64 path %.2e  bar %.1e" % (k, mm[k], mp[k], bar), flush=True)
-        if k == "eig_rel_scaled" and cond >= 1e6:
-            continue
-        assert mm[k] <= max(bar, 2.0 * mp[k]), k
-    if cond >= 1e6:
-        R = oracle.rayleigh_ld(A, B, Xo)
-        em, ep, eo = (oracle.eig_err_scaled(w, R) for w in (W, to_host(W2), Wo))
-        print("  eig vs long-double Rayleigh quotients: mixed %.2e  fp64 path %.2e  oracle %.2e  bar %.1e"
-              % (em, ep, eo, EPS * cond), flush=True)
-        assert em <= EPS * cond and ep <= EPS * cond
+    compare_to_fp64_path(oracle, A, B, 54, cond, W, X, to_host(W2), to_host(U2), "generic n=54 cond %.0e" % cond)
     if cond <= 1e4:
         assert fb == 0
+
+
+def spinwave_with_cond(geneig, batch, seed, cond, device):
+    """Spin-wave pairs (eta, H) whose H is shifted so that cond(H) = cond:
+    H + (lambda_max / cond - lambda_min) I, the extremes from numpy's eigvalsh."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, batch, seed=seed)
+    Ah, Bh = to_host(A), to_host(B)
+    lam = np.linalg.eigvalsh(np.swapaxes(Bh, -1, -2))
+    shift = lam[:, -1] / cond - lam[:, 0]
+    Bh = Bh + shift[:, None, None] * np.eye(54)[None]
+    lam2 = np.linalg.eigvalsh(np.swapaxes(Bh, -1, -2))
+    got = lam2[:, -1] / lam2[:, 0]
+    assert np.all(np.abs(got / cond - 1.0) < 1e-2), got
+    return Ah, Bh
+
+
+@pytest.mark.parametrize("cond", [1e6, 1e8])
+def test_mixed_spinwave_condition_sweep(geneig, oracle, device, cond):
+    """The n = 54 spin-wave family driven past its natural cond(B) (<~ 1e4) to
+    1e6 and 1e8 by a smaller diagonal shift (lambda_min(H) = lambda_max / cond):
+    the precision form against the oracle and the FP64 path, same rules as the
+    generic sweep."""
+    import torch
+
+    A, B = spinwave_with_cond(geneig, 48, 21, cond, device)
+    W, X, info, iters, fb = mixed(geneig, A, B, device)
+    assert (info == 0).all()
+    W2, U2, i2 = geneig.empty_outputs(54, 48, device)
+    geneig.full_n54_batched(to_dev(A, device), to_dev(B, device), W2, U2, i2)
+    torch.cuda.synchronize()
+    assert int(i2.abs().max()) == 0
+    print("spin-wave cond %.0e: fallback %d of 48, refinement steps %s" % (cond, fb, hist(iters)), flush=True)
+    compare_to_fp64_path(oracle, A, B, 54, cond, W, X, to_host(W2), to_host(U2), "spin-wave cond %.0e" % cond)
 
 
 @pytest.mark.parametrize("batch", [0, 1, 7])
@@ -164,13 +210,12 @@ def test_runtime_precision_n54_matches_static(geneig, oracle, device):
         check_pairs(oracle, Ah, Bh, to_host(W), to_host(U), Wo, Xo, 54, "precision n=54 %s" % tag)
 
 
-@pytest.mark.parametrize("n,cond", [(96, 0), (96, 1e4), (96, 1e8), (16, 0), (33, 0), (64, 1e2)])
+@pytest.mark.parametrize("n,cond", [(96, 0), (96, 1e2), (96, 1e4), (96, 1e6), (96, 1e8), (16, 0), (33, 0),
+                                    (64, 1e2)])
 def test_runtime_precision_vs_oracle(geneig, oracle, device, n, cond):
     """Precision form through the runtime solver at other n, generic pairs and
-    controlled cond(B); same bar as the n = 54 sweep."""
+    controlled cond(B); same rules as the n = 54 sweep."""
     import torch
-
-    from test_gpu_parity import EPS, TOL_EIG, TOL_PHASE
 
     if cond:
         A, B = oracle.gen_generic_cond(n, 48, seed=5, cond=cond)
@@ -182,17 +227,12 @@ def test_runtime_precision_vs_oracle(geneig, oracle, device, n, cond):
     fb = s.launch_mixed(to_dev(A, device), to_dev(B, device), W, U, info, iters=iters)
     torch.cuda.synchronize()
     assert (to_host(info) == 0).all()
-    Wo, Xo, io = oracle.zhegv(A, B)
     W2, U2, i2 = geneig.empty_outputs(n, 48, device)
     geneig.GeneigSolver(n).launch(to_dev(A, device), to_dev(B, device), W2, U2, i2)
     torch.cuda.synchronize()
-    mp = oracle.compare(A, B, to_host(W2), to_host(U2), Wo, Xo)
-    mm = oracle.compare(A, B, to_host(W), to_host(U), Wo, Xo)
     print("n=%d cond %.0e: fallback %d of 48, steps %s" % (n, cond, fb, hist(to_host(iters))), flush=True)
-    bars = {"eig_rel_scaled": TOL_EIG, "phase_err": TOL_PHASE, "resid": 50 * n * EPS, "orth": 50 * n * EPS}
-    for k, bar in bars.items():
-        print("  %-15s mixed %.2e  fp64 path %.2e  bar %.1e" % (k, mm[k], mp[k], bar), flush=True)
-        assert mm[k] <= max(bar, 2.0 * mp[k]), k
+    compare_to_fp64_path(oracle, A, B, n, cond, to_host(W), to_host(U), to_host(W2), to_host(U2),
+                         "n=%d cond %.0e" % (n, cond))
     if cond <= 1e4:
         assert fb == 0
 

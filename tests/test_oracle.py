@@ -319,3 +319,16 @@ def test_rayleigh_ld_pinned(oracle):
     Rp = oracle.rayleigh_ld(A, B, P)
     d = oracle.eig_err_scaled(Rp, Wo)
     assert 1e-15 < d < 1e-9, d
+
+
+@pytest.mark.parametrize("n,cond", [(16, 1e2), (54, 1e4), (54, 1e8), (96, 1e6)])
+def test_gen_generic_cond_hits_the_requested_condition(oracle, n, cond):
+    """B = Q diag(logspace(0, log10 cond)) Q^H with Q unitary: cond(B) is the
+    requested value (to rounding of the construction), its eigenvalues span
+    [1, cond], and A stays Hermitian."""
+    A, B = oracle.gen_generic_cond(n, 4, seed=9, cond=cond)
+    lam = np.linalg.eigvalsh(np.swapaxes(B, -1, -2))
+    got = lam[:, -1] / lam[:, 0]
+    assert np.all(np.abs(got / cond - 1.0) < 1e-6 * max(1.0, cond * 1e-8)), got
+    assert np.allclose(lam[:, 0], 1.0, rtol=1e-6) and np.allclose(lam[:, -1], cond, rtol=1e-6)
+    assert np.abs(A - np.conj(np.swapaxes(A, -1, -2))).max() == 0.0
