@@ -888,3 +888,39 @@ uint64_t go_fnv1a64(const void* data, uint64_t len, uint64_t seed) {
     }
     return h;
 }
+
+/* Rayleigh quotients x_j^H A x_j / x_j^H B x_j of every column of X, in long
+ * double (64-bit significand), for a batch stored column-major like the
+ * solver's data: the batched form of oracle.rayleigh_ld (same sums, in C, one
+ * pair per OpenMP thread) for checking many pairs at once.  A checker with no
+ * reference counterpart. */
+void go_rayleigh_ld_batch(int n, int batch, const go_cplx* A, const go_cplx* B, const go_cplx* X, double* out,
+                          int nthreads) {
+    const size_t nn = (size_t)n * n;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+    for (int p = 0; p < batch; ++p) {
+        const go_cplx* Ap = A + nn * p;
+        const go_cplx* Bp = B + nn * p;
+        const go_cplx* Xp = X + nn * p;
+        for (int j = 0; j < n; ++j) {
+            const go_cplx* x = Xp + (size_t)j * n;
+            long double num = 0.0L, den = 0.0L;
+            for (int i = 0; i < n; ++i) {
+                long double ar = 0.0L, ai = 0.0L, br = 0.0L, bi = 0.0L;
+                for (int k = 0; k < n; ++k) {
+                    const long double xr = creal(x[k]), xi = cimag(x[k]);
+                    const go_cplx a = Ap[i + (size_t)k * n], b = Bp[i + (size_t)k * n];
+                    ar += (long double)creal(a) * xr - (long double)cimag(a) * xi;
+                    ai += (long double)creal(a) * xi + (long double)cimag(a) * xr;
+                    br += (long double)creal(b) * xr - (long double)cimag(b) * xi;
+                    bi += (long double)creal(b) * xi + (long double)cimag(b) * xr;
+                }
+                const long double cr = creal(x[i]), ci = cimag(x[i]);   /* conj(x_i) (y_i) */
+                num += cr * ar + ci * ai;
+                den += cr * br + ci * bi;
+            }
+            out[(size_t)p * n + j] = (double)(num / den);
+        }
+    }
+}

@@ -136,6 +136,11 @@ int go_geig_write(const char* path, int n, int batch, int seed,
 /* FNV-1a 64 (known-answer anchor for the module-cache key hashing). */
 uint64_t go_fnv1a64(const void* data, uint64_t len, uint64_t seed);
 
+/* Long-double Rayleigh quotients of every column of X (batch x n out): the
+ * batched form of oracle.rayleigh_ld. */
+void go_rayleigh_ld_batch(int n, int batch, const go_cplx* A, const go_cplx* B, const go_cplx* X, double* out,
+                          int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,7 @@ def lib():
         "go_geig_read": (i32, [C.c_char_p, i32, i32, P, P, P, P]),
         "go_geig_write": (i32, [C.c_char_p, i32, i32, i32, P, P, P, P]),
         "go_fnv1a64": (u64, [P, u64, u64]),
+        "go_rayleigh_ld_batch": (None, [i32, i32, P, P, P, P, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -298,6 +299,15 @@ def rayleigh_ld(A, B, X):
         num = (Xm.conj() * (Am @ Xm)).sum(axis=0).real
         den = (Xm.conj() * (Bm @ Xm)).sum(axis=0).real
         out[p] = (num / den).astype(np.float64)
+    return out
+
+
+def rayleigh_ld_batch(A, B, X, nthreads=0):
+    """rayleigh_ld in C (long double, one pair per OpenMP thread): the same
+    quotients for many pairs at once (batch x n, float64)."""
+    A, B, X = _c128(A), _c128(B), _c128(X)
+    out = np.empty(X.shape[:2], np.float64)
+    lib().go_rayleigh_ld_batch(X.shape[-1], X.shape[0], _p(A), _p(B), _p(X), _p(out), int(nthreads))
     return out
 
 

@@ -332,3 +332,21 @@ def test_gen_generic_cond_hits_the_requested_condition(oracle, n, cond):
     assert np.all(np.abs(got / cond - 1.0) < 1e-6 * max(1.0, cond * 1e-8)), got
     assert np.allclose(lam[:, 0], 1.0, rtol=1e-6) and np.allclose(lam[:, -1], cond, rtol=1e-6)
     assert np.abs(A - np.conj(np.swapaxes(A, -1, -2))).max() == 0.0
+
+
+def test_rayleigh_ld_batch_equals_python_form(oracle):
+    """The C batched long-double Rayleigh quotients agree with rayleigh_ld's
+    numpy long-double evaluation (the same sums) to a few ulps of FP64, and on
+    a diagonal pair equal the exact diagonal ratios."""
+    A, B = oracle.gen_generic_cond(24, 6, seed=13, cond=1e6)
+    W, X, info = oracle.zhegv(A, B)
+    R1 = oracle.rayleigh_ld(A, B, X)
+    R2 = oracle.rayleigh_ld_batch(A, B, X, nthreads=2)
+    assert np.abs(R1 - R2).max() <= 4 * np.finfo(np.float64).eps * np.abs(R1).max()
+    n = 5
+    d = np.array([1.0, -2.0, 3.5, 0.25, 7.0])
+    b = np.array([2.0, 1.0, 4.0, 0.5, 3.0])
+    Ad = np.diag(d).astype(np.complex128)[None]
+    Bd = np.diag(b).astype(np.complex128)[None]
+    Xd = np.eye(n, dtype=np.complex128)[None] * (1.0 + 2.0j)
+    assert np.array_equal(oracle.rayleigh_ld_batch(Ad, Bd, Xd)[0], d / b)
