@@ -5,10 +5,18 @@
 // void* (hipDeviceptr_t) instead of CUdeviceptr and streams are hipStream_t.
 // Errors throw std::runtime_error carrying the library's message, as the
 // reference does (src/nvrtc/nvrtc_solver.cpp:49-91).
+//
+// The classes live in namespace geneig.  The reference's own spellings
+// (NvrtcGeneigSolver, TuningMode, CacheMode, ModuleCacheKey, make_streams,
+// destroy_streams, all at global scope: nvrtc_solver.hpp:27-79) are declared
+// at the end as aliases, so a caller written against the reference compiles
+// once its CUDA types are replaced by HIP ones; define
+// GENEIG_NO_REFERENCE_NAMES to leave the global namespace alone.
 #pragma once
 
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "geneig_solver.h"
@@ -18,18 +26,36 @@ namespace geneig {
 enum class TuningMode { Suggested = GENEIG_TUNING_SUGGESTED, Autotune = GENEIG_TUNING_AUTOTUNE };
 enum class CacheMode { Auto = GENEIG_CACHE_AUTO, NoDisk = GENEIG_CACHE_NODISK, NoCache = GENEIG_CACHE_NOCACHE };
 
-struct Dim3 {
-    unsigned x = 0, y = 0, z = 0;
-};
-
 // Cache key of one compiled module (nvrtc_solver.hpp:67-79): two solvers with
-// equal keys share one loaded module.
+// equal keys share one loaded module.  arch is the gfx target number, 950 for
+// gfx950 (the reference's int compute capability, 90 for sm_90); variant
+// selects the module (0 the FP64 solve, 1 the precision-form one) and is not
+// part of the reference's key.
 struct ModuleCacheKey {
     int n = 0;
     int bpb = 0;
-    std::string arch;
+    int arch = 0;
     int block_dim_x = 64;
     int variant = 0;
+    bool operator<(const ModuleCacheKey& o) const {
+        return std::tie(n, bpb, arch, block_dim_x, variant) < std::tie(o.n, o.bpb, o.arch, o.block_dim_x, o.variant);
+    }
+    // The target name the library keys its modules on: the full name of a
+    // visible device of this target (gfx950:sramecc+:xnack-, features
+    // included, since code objects are built for them), else gfx<arch>.
+    std::string arch_name() const {
+        const std::string base = "gfx" + std::to_string(arch);
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess) return base;
+        for (int d = 0; d < count; ++d) {
+            hipDeviceProp_t p;
+            if (hipGetDeviceProperties(&p, d) != hipSuccess) continue;
+            const std::string full = p.gcnArchName;
+            if (full.compare(0, base.size(), base) == 0 && (full.size() == base.size() || full[base.size()] == ':'))
+                return full;
+        }
+        return base;
+    }
 };
 
 inline void check(int rc, const char* what) {
@@ -74,10 +100,10 @@ public:
 
     int matrix_size() const { return geneig_solver_matrix_size(h_); }
     int batches_per_block() const { return geneig_solver_batches_per_block(h_); }
-    Dim3 block_dim() const {
-        Dim3 d;
-        geneig_solver_block_dim(h_, &d.x, &d.y, &d.z);
-        return d;
+    dim3 block_dim() const {
+        unsigned x = 0, y = 0, z = 0;
+        geneig_solver_block_dim(h_, &x, &y, &z);
+        return dim3(x, y, z);
     }
     unsigned int shared_mem_bytes() const { return geneig_solver_shared_mem_bytes(h_); }
     int autotuned_block_dim_x() const { return geneig_solver_autotuned_block_dim_x(h_); }
@@ -92,11 +118,12 @@ public:
         s.resize(s.size() - 1);
         return s;
     }
+    // The on-disk code-object path of a key (the reference names it a cubin).
     static std::string cubin_path(const ModuleCacheKey& key) {
-        const size_t len = geneig_cache_path(key.n, key.bpb, key.arch.c_str(), key.block_dim_x, key.variant,
-                                             nullptr, 0);
+        const std::string arch = key.arch_name();
+        const size_t len = geneig_cache_path(key.n, key.bpb, arch.c_str(), key.block_dim_x, key.variant, nullptr, 0);
         std::string s(len + 1, '\0');
-        geneig_cache_path(key.n, key.bpb, key.arch.c_str(), key.block_dim_x, key.variant, &s[0], s.size());
+        geneig_cache_path(key.n, key.bpb, arch.c_str(), key.block_dim_x, key.variant, &s[0], s.size());
         s.resize(len);
         return s;
     }
@@ -117,3 +144,13 @@ inline void destroy_streams(std::vector<hipStream_t>& streams) {
 }
 
 }  // namespace geneig
+
+#ifndef GENEIG_NO_REFERENCE_NAMES
+// The reference's global names (nvrtc_solver.hpp:27-79, 81).
+using NvrtcGeneigSolver = geneig::GeneigSolver;
+using TuningMode = geneig::TuningMode;
+using CacheMode = geneig::CacheMode;
+using ModuleCacheKey = geneig::ModuleCacheKey;
+using geneig::destroy_streams;
+using geneig::make_streams;
+#endif

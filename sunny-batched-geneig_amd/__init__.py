@@ -17,7 +17,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgeneig.so")
+# GENEIG_LIBRARY names another build of the library (the host-sanitized one
+# of ``make SANITIZE=1``); by default the in-tree libgeneig.so is loaded.
+LIB_PATH = os.environ.get("GENEIG_LIBRARY") or os.path.join(HERE, "libgeneig.so")
 
 TUNING_SUGGESTED, TUNING_AUTOTUNE = 0, 1
 CACHE_AUTO, CACHE_NODISK, CACHE_NOCACHE = 0, 1, 2
