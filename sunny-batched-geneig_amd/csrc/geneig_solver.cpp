@@ -214,6 +214,7 @@ struct Module {
     hipFunction_t fr = nullptr, f1l = nullptr, f2al = nullptr, f2bl = nullptr, f2xl = nullptr, f3l = nullptr;
     hipFunction_t fz = nullptr;            // clears device counters (both variants)
     unsigned f_lds_per_pair = 0, f_k2_threads = 0, refine_lds = 0, refine_threads = 0;
+    unsigned f_lds_total = 0;      // larger of K1f / K3f dynamic LDS + static vectors, per pair
     unsigned long long f_ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
     Module() = default;
@@ -272,6 +273,7 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     if (!rc) rc = read_global("geneig_chol64_lds_bytes", &m->chol64_lds, sizeof(unsigned));
     if (variant == 1) {
         if (!rc) rc = read_global("geneig_f_lds_bytes_per_pair", &m->f_lds_per_pair, sizeof(unsigned));
+        if (!rc) rc = read_global("geneig_f_lds_total_per_pair", &m->f_lds_total, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2_threads", &m->f_k2_threads, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k2a_lds_bytes", &m->f_k2a_lds, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_k3_lds_bytes", &m->f_k3_lds, sizeof(unsigned));
@@ -895,7 +897,9 @@ int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int c
     s->bpb = 1;
     s->variant = 1;
     const Module& m = *s->module;
-    s->shared_bytes = std::max(m.lds_per_pair, std::max(m.f_lds_per_pair, m.refine_lds));
+    // every kernel of the launch must fit, static broadcast vectors included:
+    // the FP64 ones re-solve the pairs the refinement hands back
+    s->shared_bytes = std::max(m.lds_total, std::max(m.f_lds_total, m.refine_lds));
     if (s->shared_bytes > s->lds_per_block)
         return fail(GENEIG_E_FIT, "n=%d: precision form needs %u B LDS per workgroup > %zu", n, s->shared_bytes,
                     s->lds_per_block);

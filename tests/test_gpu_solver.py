@@ -224,3 +224,32 @@ print("ok", m["eig_rel_scaled"])
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
     print(out.stdout, out.stderr[-2000:], flush=True)
     assert out.returncode == 0 and "ok" in out.stdout
+
+
+def test_largest_size_solves_and_larger_sizes_are_refused(geneig, oracle, device, cache_dir):
+    """Above n = 64 every stage keeps the pair's n x LD working matrix in LDS
+    (16 n LD bytes in FP64, LD = n + 1 for even n, plus the two broadcast
+    vectors): n = 98 needs 159,840 B and fits the 160 KB a workgroup may use,
+    n = 100 needs 166,208 B and does not.  The reference's multi-N test goes to
+    n = 128 (tests/test_nvrtc_multiN.cpp:218-271); here n = 98 solves to the
+    FP64 bars in both forms, and n = 100 and 128 are refused at construction
+    with GENEIG_E_FIT (code 4) for both forms (the precision form keeps the FP64
+    kernels for the pairs its refinement hands back)."""
+    from test_gpu_parity import check_pairs
+
+    n = 98
+    A, B = geneig.gen_generic(n, 16, seed=23)
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    Wo, Xo, io = oracle.zhegv(Ah, Bh)
+    assert (io == 0).all()
+    for tag, s in (("fp64", geneig.GeneigSolver(n)), ("precision", geneig.GeneigMixedSolver(n))):
+        print("n=%d %s: %d B LDS per workgroup" % (n, tag, s.shared_mem_bytes()), flush=True)
+        W, X, info = run(geneig, s, A, B, device)
+        assert (info == 0).all()
+        check_pairs(oracle, Ah, Bh, W, X, Wo, Xo, n, "n=%d %s" % (n, tag))
+    for big in (100, 128):
+        for make in (geneig.GeneigSolver, geneig.GeneigMixedSolver):
+            with pytest.raises(geneig.GeneigError) as e:
+                make(big)
+            print("n=%d %s refused: %s" % (big, make.__name__, e.value), flush=True)
+            assert "(code 4)" in str(e.value) and "LDS" in str(e.value)
