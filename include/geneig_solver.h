@@ -98,6 +98,12 @@ const char* geneig_solver_cache_layer_used(geneig_solver_t s); /* "in-process"|"
 double geneig_solver_acquire_module_ms(geneig_solver_t s);
 /* Eigen-algorithm variant of the loaded module (0 = tridiagonal QL). */
 int geneig_solver_variant(geneig_solver_t s);
+/* Rows per chunk of K1's register walks in the loaded module (n <= 64; 0
+ * above, where K1 works in LDS): the tile-shape knob the autotuner sweeps (4,
+ * 8 or 16 at the winning pairs per workgroup), else the build's default (8 in
+ * FP64, 16 in complex64).  No counterpart in the reference, whose autotuner
+ * sweeps threads per pair (nvrtc_solver.cpp:649-925). */
+int geneig_solver_k1_chunk_rows(geneig_solver_t s);
 
 /* NvrtcGeneigSolver::cache_dir / cubin_path (nvrtc_solver.cpp:403-431).
  * Copies a NUL-terminated path into buf; returns the full length. */
@@ -110,6 +116,10 @@ size_t geneig_cache_path(int n, int bpb, const char* arch, int block_dim_x, int 
  * size, and when buf != NULL and cap >= size the object is copied there. */
 int geneig_rtc_compile(int n, int bpb, int variant, const char* arch, void* buf, size_t cap,
                        size_t* size);
+/* The same with K1's row-chunk width set (4, 8 or 16; 0 = the build's
+ * default), as the autotuner's candidates are built. */
+int geneig_rtc_compile_rows(int n, int bpb, int variant, int k1_rows, const char* arch, void* buf, size_t cap,
+                            size_t* size);
 uint64_t geneig_source_hash(void);
 uint64_t geneig_fnv1a64(const void* data, size_t len, uint64_t seed);
 

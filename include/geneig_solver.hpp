@@ -107,6 +107,9 @@ public:
     }
     unsigned int shared_mem_bytes() const { return geneig_solver_shared_mem_bytes(h_); }
     int autotuned_block_dim_x() const { return geneig_solver_autotuned_block_dim_x(h_); }
+    // Rows per chunk of K1's register walks: the tile the autotuner picked, or
+    // the build's default (no counterpart in the reference).
+    int k1_chunk_rows() const { return geneig_solver_k1_chunk_rows(h_); }
     bool was_autotuned() const { return geneig_solver_was_autotuned(h_) != 0; }
     bool autotune_sweep_ran() const { return geneig_solver_autotune_sweep_ran(h_) != 0; }
     const char* cache_layer_used() const { return geneig_solver_cache_layer_used(h_); }

@@ -90,6 +90,7 @@ def lib():
         "geneig_solver_cache_layer_used": (C.c_char_p, [P]),
         "geneig_solver_acquire_module_ms": (D, [P]),
         "geneig_solver_variant": (I, [P]),
+        "geneig_solver_k1_chunk_rows": (I, [P]),
         "geneig_solver_reserve": (I, [P, I, P]),
         "geneig_solver_workspace_bytes": (C.c_size_t, [P, I]),
         "geneig_n54_workspace_bytes": (C.c_size_t, [I, I]),
@@ -108,6 +109,7 @@ def lib():
         "geneig_cache_dir": (C.c_size_t, [C.c_char_p, C.c_size_t]),
         "geneig_cache_path": (C.c_size_t, [I, I, C.c_char_p, I, I, C.c_char_p, C.c_size_t]),
         "geneig_rtc_compile": (I, [I, I, I, C.c_char_p, P, C.c_size_t, P]),
+        "geneig_rtc_compile_rows": (I, [I, I, I, I, C.c_char_p, P, C.c_size_t, P]),
         "geneig_source_hash": (U64, []),
         "geneig_fnv1a64": (U64, [P, C.c_size_t, U64]),
         "geneig_make_streams": (I, [I, P]),
@@ -407,6 +409,10 @@ class GeneigSolver:
     def variant(self):
         return lib().geneig_solver_variant(self._h)
 
+    def k1_chunk_rows(self):
+        """Rows per chunk of K1's register walks (the autotuned tile; 0 above n = 64)."""
+        return lib().geneig_solver_k1_chunk_rows(self._h)
+
     def reserve(self, batch, stream=None):
         """Pre-allocate the workspace a launch of `batch` pairs on `stream` uses."""
         _check(lib().geneig_solver_reserve(self._h, int(batch), _stream(stream)), "GeneigSolver.reserve")
@@ -502,14 +508,14 @@ class GeneigMulti:
     def replays(self):
         return lib().geneig_multi_replays(self._h)
 
-def rtc_compile(n, bpb=1, variant=0, arch="gfx950"):
-    """Device-free hipRTC compile of the per-n kernel; returns the code object bytes."""
+def rtc_compile(n, bpb=1, variant=0, arch="gfx950", k1_rows=0):
+    """Device-free hipRTC compile of the per-n kernel; returns the code object
+    bytes.  k1_rows: K1's row-chunk width (4, 8, 16; 0 = the build's default)."""
     size = C.c_size_t(0)
-    _check(lib().geneig_rtc_compile(int(n), int(bpb), int(variant), arch.encode(), None, 0, C.byref(size)),
-           "rtc_compile(n=%d)" % n)
+    args = (int(n), int(bpb), int(variant), int(k1_rows), arch.encode())
+    _check(lib().geneig_rtc_compile_rows(*args, None, 0, C.byref(size)), "rtc_compile(n=%d)" % n)
     buf = C.create_string_buffer(size.value)
-    _check(lib().geneig_rtc_compile(int(n), int(bpb), int(variant), arch.encode(), buf, size.value,
-                                    C.byref(size)), "rtc_compile(n=%d)" % n)
+    _check(lib().geneig_rtc_compile_rows(*args, buf, size.value, C.byref(size)), "rtc_compile(n=%d)" % n)
     return buf.raw[: size.value]
 
 

@@ -112,10 +112,14 @@ std::string cache_dir_str() {
     return "/tmp/sunny_geneig/hsaco";
 }
 
-std::string cache_path_str(int n, int bpb, const std::string& arch, int bdx, int variant) {
+// k1_rows: K1's row-chunk width when the autotuner set one (0: the build's
+// default, and the reference's path format).
+std::string cache_path_str(int n, int bpb, const std::string& arch, int bdx, int variant, int k1_rows = 0) {
+    char rows[16] = "";
+    if (k1_rows > 0) std::snprintf(rows, sizeof rows, "_r%d", k1_rows);
     char buf[1024];
-    std::snprintf(buf, sizeof buf, "%s/%d_%d_%s_%d_v%d_lib%016llx_src%016llx.hsaco",
-                  cache_dir_str().c_str(), n, bpb, sanitize(arch).c_str(), bdx, variant,
+    std::snprintf(buf, sizeof buf, "%s/%d_%d_%s_%d_v%d%s_lib%016llx_src%016llx.hsaco",
+                  cache_dir_str().c_str(), n, bpb, sanitize(arch).c_str(), bdx, variant, rows,
                   (unsigned long long)toolchain_hash(), (unsigned long long)source_hash());
     return buf;
 }
@@ -159,10 +163,12 @@ bool write_atomic(const std::string& path, const void* data, size_t size) {
 }
 
 // ---- hipRTC ----------------------------------------------------------------
-int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vector<char>& code) {
+int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vector<char>& code, int k1_rows = 0) {
     if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
     if (bpb < 1 || bpb > 16) return fail(GENEIG_E_ARG, "pairs per workgroup %d outside 1..16", bpb);
     if (variant != 0 && variant != 1) return fail(GENEIG_E_ARG, "variant %d is not built", variant);
+    if (k1_rows != 0 && k1_rows != 4 && k1_rows != 8 && k1_rows != 16)
+        return fail(GENEIG_E_ARG, "K1 row chunk %d is not 4, 8 or 16", k1_rows);
     hiprtcProgram prog = nullptr;
     if (hiprtcCreateProgram(&prog, kGeneigDeviceSource, "geneig_device.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
@@ -170,6 +176,7 @@ int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vecto
     std::vector<std::string> opts = {"--offload-arch=" + arch, "-DGENEIG_RTC_N=" + std::to_string(n),
                                      "-DGENEIG_RTC_BPB=" + std::to_string(bpb),
                                      "-DGENEIG_RTC_VARIANT=" + std::to_string(variant)};
+    if (k1_rows > 0) opts.push_back("-DGENEIG_RTC_K1_RCH=" + std::to_string(k1_rows));
     for (const char* f : kRtcFlags) opts.push_back(f);
     std::vector<const char*> p;
     for (auto& s : opts) p.push_back(s.c_str());
@@ -289,8 +296,9 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
 struct Key {
     int n, bpb, variant, bdx;
     std::string arch;
+    int k1_rows;
     bool operator<(const Key& o) const {
-        return std::tie(n, bpb, variant, bdx, arch) < std::tie(o.n, o.bpb, o.variant, o.bdx, o.arch);
+        return std::tie(n, bpb, variant, bdx, arch, k1_rows) < std::tie(o.n, o.bpb, o.variant, o.bdx, o.arch, o.k1_rows);
     }
 };
 
@@ -298,8 +306,13 @@ std::mutex g_module_mu;
 std::map<Key, std::weak_ptr<Module>> g_modules;
 
 struct TuneWinner {
-    int bpb = 0, variant = 0;
+    int bpb = 0, variant = 0, k1_rows = 0;
     double pairs_per_s = 0.0;
+};
+// One autotune candidate: pairs per workgroup, module variant, K1 row chunk
+// (0: the build's default).
+struct TuneCand {
+    int bpb, variant, k1_rows;
 };
 std::mutex g_tune_mu;
 std::map<std::pair<int, std::string>, TuneWinner> g_tune;
@@ -360,6 +373,7 @@ struct geneig_solver {
     std::shared_ptr<Module> probe;
     unsigned shared_bytes = 0;
     int autotuned_bdx = 0;
+    int k1_rows = 0;               // K1 row-chunk width of the module (0: the build's default)
     bool was_autotuned = false, sweep_ran = false;
     const char* layer = "compile";
     double acquire_ms = 0.0;
@@ -400,8 +414,8 @@ struct geneig_solver {
 
     // Module acquisition through the cache layers (nvrtc_solver.cpp:522-597).
     int acquire(int want_bpb, int want_variant, int cache_mode, std::shared_ptr<Module>& out,
-                const char** layer_out) {
-        const Key key{n, want_bpb, want_variant, 64, arch};
+                const char** layer_out, int k1_rows = 0) {
+        const Key key{n, want_bpb, want_variant, 64, arch, k1_rows};
         if (cache_mode != GENEIG_CACHE_NOCACHE) {
             std::lock_guard<std::mutex> lk(g_module_mu);
             auto it = g_modules.find(key);
@@ -414,7 +428,7 @@ struct geneig_solver {
                 g_modules.erase(it);
             }
         }
-        const std::string path = cache_path_str(n, want_bpb, arch, 64, want_variant);
+        const std::string path = cache_path_str(n, want_bpb, arch, 64, want_variant, k1_rows);
         if (cache_mode == GENEIG_CACHE_AUTO) {
             std::ifstream f(path, std::ios::binary | std::ios::ate);
             if (f) {
@@ -439,7 +453,7 @@ struct geneig_solver {
             }
         }
         std::vector<char> code;
-        int rc = rtc_compile(n, want_bpb, want_variant, arch, code);
+        int rc = rtc_compile(n, want_bpb, want_variant, arch, code, k1_rows);
         if (rc) return rc;
         std::shared_ptr<Module> h;
         rc = load_module(code, n, want_bpb, want_variant, h);
@@ -640,11 +654,10 @@ struct geneig_solver {
                                   (int)m.chol64_lds);
     }
 
-    int autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands, TuneWinner& win);
+    int autotune(int cache_mode, const std::vector<TuneCand>& cands, TuneWinner& win);
 };
 
-int geneig_solver::autotune(int cache_mode, const std::vector<std::pair<int, int>>& cands,
-                            TuneWinner& win) {
+int geneig_solver::autotune(int cache_mode, const std::vector<TuneCand>& cands, TuneWinner& win) {
     // Workload and validation follow src/nvrtc/nvrtc_solver.cpp:649-925: 1024
     // pairs, 5-sample median timing, 8 spot pairs checked against the first
     // candidate at 1e-10 (eigenvalues scaled by max(|w|,1), and 1 - |u^H B u_ref|).
@@ -674,11 +687,11 @@ int geneig_solver::autotune(int cache_mode, const std::vector<std::pair<int, int
     (void)hipEventCreate(&e1);
     bool have = false;
     for (size_t ci = 0; ci < cands.size(); ++ci) {
-        const int b = cands[ci].first, v = cands[ci].second;
+        const int b = cands[ci].bpb, v = cands[ci].variant, rows = cands[ci].k1_rows;
         std::shared_ptr<Module> m;
         const char* lay = nullptr;
-        if (acquire(b, v, cache_mode, m, &lay) != GENEIG_E_OK) {
-            std::fprintf(stderr, "[autotune] bpb=%d variant=%d: rejected at compile/load\n", b, v);
+        if (acquire(b, v, cache_mode, m, &lay, rows) != GENEIG_E_OK) {
+            std::fprintf(stderr, "[autotune] bpb=%d variant=%d k1_rows=%d: rejected at compile/load\n", b, v, rows);
             continue;
         }
         const unsigned bytes = m->lds_total * (unsigned)b;
@@ -731,11 +744,13 @@ int geneig_solver::autotune(int cache_mode, const std::vector<std::pair<int, int
             }
             ok = eig < tol && phase < tol;
         }
-        std::fprintf(stderr, "[autotune] n=%d bpb=%d variant=%d: median %.0f pairs/s eig=%.2e phase=%.2e %s\n",
-                     n, b, v, mps, eig, phase, ok ? "[OK]" : "[REJECT correctness]");
+        std::fprintf(stderr,
+                     "[autotune] n=%d bpb=%d variant=%d k1_rows=%d: median %.0f pairs/s eig=%.2e phase=%.2e %s\n",
+                     n, b, v, rows, mps, eig, phase, ok ? "[OK]" : "[REJECT correctness]");
         if (ok && mps > win.pairs_per_s) {
             win.bpb = b;
             win.variant = v;
+            win.k1_rows = rows;
             win.pairs_per_s = mps;
         }
     }
@@ -793,8 +808,11 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
         }
         if (!hit && cache_mode == GENEIG_CACHE_AUTO) {
             std::ifstream f(tune_path_str(n, s->arch));
-            if (f >> win.bpb >> win.variant >> win.pairs_per_s && win.bpb >= 1 && win.bpb <= 16) hit = true;
-            else win = TuneWinner();
+            if (f >> win.bpb >> win.variant >> win.pairs_per_s >> win.k1_rows && win.bpb >= 1 && win.bpb <= 16 &&
+                (win.k1_rows == 0 || win.k1_rows == 4 || win.k1_rows == 8 || win.k1_rows == 16))
+                hit = true;
+            else
+                win = TuneWinner();
         }
         if (bpb_request > 0 && hit && win.bpb != bpb_request) hit = false;
         if (!hit) {
@@ -805,22 +823,35 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
             if (rc) return rc;
             note(lay);
             keep = m1;
-            std::vector<std::pair<int, int>> cands;
+            // stage 1: pairs per workgroup with the default tile; stage 2: the
+            // K1 row-chunk width (the tile shape of its register walks) at the
+            // winning pairs per workgroup, the stage-1 winner first as the
+            // correctness reference (n <= 64: the walks exist only there)
+            std::vector<TuneCand> cands;
             for (int b : {1, 2, 4, 8})
-                if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_total)) cands.push_back({b, 0});
+                if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_total)) cands.push_back({b, 0, 0});
             if (cands.empty()) return fail(GENEIG_E_FIT, "n=%d: no pairs-per-workgroup candidate fits", n);
             rc = s->autotune(cache_mode, cands, win);
             if (rc) return rc;
+            if (n <= 64) {
+                const TuneWinner first = win;
+                std::vector<TuneCand> tiles = {{first.bpb, 0, 0}, {first.bpb, 0, 4}, {first.bpb, 0, 16}};
+                win = TuneWinner();
+                rc = s->autotune(cache_mode, tiles, win);
+                if (rc) return rc;
+            }
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{n, s->arch}] = win;
             if (cache_mode == GENEIG_CACHE_AUTO) {
                 char line[128];
-                int len = std::snprintf(line, sizeof line, "%d %d %.1f\n", win.bpb, win.variant, win.pairs_per_s);
+                int len = std::snprintf(line, sizeof line, "%d %d %.1f %d\n", win.bpb, win.variant, win.pairs_per_s,
+                                        win.k1_rows);
                 write_atomic(tune_path_str(n, s->arch), line, (size_t)len);
             }
         }
         bpb = win.bpb;
         variant = win.variant;
+        s->k1_rows = win.k1_rows;
         s->was_autotuned = true;
         s->autotuned_bdx = 64;
     }
@@ -837,7 +868,7 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
                         m1->lds_total, s->lds_per_block);
     }
     const char* lay = "compile";
-    int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay);
+    int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay, s->k1_rows);
     if (rc) return rc;
     note(lay);
     s->probe = std::move(keep);
@@ -996,6 +1027,11 @@ int geneig_solver_autotune_sweep_ran(geneig_solver_t s) { return s ? (int)s->swe
 const char* geneig_solver_cache_layer_used(geneig_solver_t s) { return s ? s->layer : ""; }
 double geneig_solver_acquire_module_ms(geneig_solver_t s) { return s ? s->acquire_ms : 0.0; }
 int geneig_solver_variant(geneig_solver_t s) { return s ? s->variant : -1; }
+int geneig_solver_k1_chunk_rows(geneig_solver_t s) {
+    if (!s || s->n > 64) return 0;     // above n = 64 K1 works in LDS, without register walks
+    if (s->k1_rows > 0) return s->k1_rows;
+    return s->variant == 1 ? 16 : 8;   // the defaults of k1_rch<T>(): FP64 walks 8 rows, complex64 16
+}
 
 size_t geneig_cache_dir(char* buf, size_t len) {
     const std::string d = cache_dir_str();
@@ -1011,8 +1047,13 @@ size_t geneig_cache_path(int n, int bpb, const char* arch, int block_dim_x, int 
 }
 
 int geneig_rtc_compile(int n, int bpb, int variant, const char* arch, void* buf, size_t cap, size_t* size) {
+    return geneig_rtc_compile_rows(n, bpb, variant, 0, arch, buf, cap, size);
+}
+
+int geneig_rtc_compile_rows(int n, int bpb, int variant, int k1_rows, const char* arch, void* buf, size_t cap,
+                            size_t* size) {
     std::vector<char> code;
-    int rc = rtc_compile(n, bpb, variant, arch ? arch : "gfx950", code);
+    int rc = rtc_compile(n, bpb, variant, arch ? arch : "gfx950", code, k1_rows);
     if (rc) return rc;
     if (size) *size = code.size();
     if (buf && cap >= code.size()) std::memcpy(buf, code.data(), code.size());

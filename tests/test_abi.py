@@ -69,6 +69,15 @@ def test_rtc_precision_module_compiles_without_device(geneig, n):
     assert code[:4] == b"\x7fELF" and len(code) > 1024
 
 
+@pytest.mark.parametrize("n,rows", [(32, 4), (32, 16), (54, 4), (54, 16), (64, 16)])
+def test_rtc_k1_row_chunks_compile_without_device(geneig, n, rows):
+    """The autotuner's tile-shape candidates (K1's row-chunk width 4 / 16
+    besides the default 8) build for n <= 64, and differ from the default."""
+    code = geneig.rtc_compile(n, 1, k1_rows=rows)
+    assert code[:4] == b"\x7fELF" and len(code) > 1024
+    assert code != geneig.rtc_compile(n, 1)
+
+
 def test_rtc_rejects_bad_arguments(geneig):
     with pytest.raises(geneig.GeneigError):
         geneig.rtc_compile(0)
@@ -76,6 +85,8 @@ def test_rtc_rejects_bad_arguments(geneig):
         geneig.rtc_compile(200)
     with pytest.raises(geneig.GeneigError):
         geneig.rtc_compile(32, 1, variant=7)
+    with pytest.raises(geneig.GeneigError, match="row chunk"):
+        geneig.rtc_compile(32, 1, k1_rows=32)        # would read past the vectors' zero pad
 
 
 def test_cache_dir_resolution(geneig, monkeypatch):

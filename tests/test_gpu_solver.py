@@ -99,14 +99,24 @@ def test_cache_invalidation_recompiles(geneig, device, cache_dir):
 
 
 def test_autotune_then_cache_hit(geneig, oracle, device, cache_dir):
-    """Sweep on first construction, winner reused after (test_autotune_n54.cpp:308-346)."""
+    """Sweep on first construction, winner reused after (test_autotune_n54.cpp:308-346).
+    The sweep picks pairs per workgroup, then K1's row-chunk width (the tile
+    shape of its register walks) at that pairs per workgroup; both are
+    persisted and reused."""
     s1 = geneig.GeneigSolver(32, tuning_mode=geneig.TUNING_AUTOTUNE)
     assert s1.was_autotuned() and s1.autotune_sweep_ran()
     assert s1.batches_per_block() in (1, 2, 4, 8)
+    assert s1.k1_chunk_rows() in (4, 8, 16)
     s2 = geneig.GeneigSolver(32, tuning_mode=geneig.TUNING_AUTOTUNE)
     assert s2.was_autotuned() and not s2.autotune_sweep_ran()
     assert s2.batches_per_block() == s1.batches_per_block()
-    assert any(f.startswith("autotune_32_") for f in os.listdir(cache_dir))
+    assert s2.k1_chunk_rows() == s1.k1_chunk_rows()
+    tune = [f for f in os.listdir(cache_dir) if f.startswith("autotune_32_")]
+    assert len(tune) == 1
+    fields = open(os.path.join(cache_dir, tune[0])).read().split()
+    print("autotune record (bpb variant pairs/s k1_rows):", fields, flush=True)
+    assert len(fields) == 4 and int(fields[0]) == s1.batches_per_block()
+    assert int(fields[3]) in (0, 4, 16) and (int(fields[3]) or 8) == s1.k1_chunk_rows()
     A, B = oracle.gen_generic(32, 16, seed=3)
     W, X, info = run(geneig, s2, A, B, device)
     Wo, Xo, _ = oracle.zhegv(A, B)
