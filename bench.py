@@ -505,8 +505,16 @@ def bench_precision_sweep(args, geneig, torch, dev):
             Ah, Bh = A.index_select(0, idx).cpu().numpy(), B.index_select(0, idx).cpu().numpy()
             Wr, Xr, ir = O.lapack_zhegv(Ah, Bh) if have_lapack else O.zhegv(Ah, Bh)
             ok = ir == 0
-            ep = eig_errors(Wp.index_select(0, idx).cpu().numpy()[ok], Wr[ok])
-            em = eig_errors(W.index_select(0, idx).cpu().numpy()[ok], Wr[ok])
+            Wph, Wmh = Wp.index_select(0, idx).cpu().numpy()[ok], W.index_select(0, idx).cpu().numpy()[ok]
+            ep = eig_errors(Wph, Wr[ok])
+            em = eig_errors(Wmh, Wr[ok])
+            # at large cond(B) every FP64 eigenvalue, LAPACK's included, carries
+            # ~eps cond(B): long-double Rayleigh quotients of the reference's
+            # eigenvectors (second order in their error) are the sharp reference
+            Rld = O.rayleigh_ld_batch(Ah[ok], Bh[ok], Xr[ok])
+            ep["vs_long_double"] = eig_errors(Wph, Rld)["max_eig_rel_err_floor1"]
+            em["vs_long_double"] = eig_errors(Wmh, Rld)["max_eig_rel_err_floor1"]
+            ref_ld = eig_errors(Wr[ok], Rld)["max_eig_rel_err_floor1"]
             v, c = np.unique(iters.cpu().numpy(), return_counts=True)
             row = {"family": fam, "n": n, "cond_B": cond, "pairs": batch,
                    "precision": {"solves_per_s": batch / tm, "ms_per_step": tm * 1e3, "fallback_rate": fb / batch,
@@ -515,11 +523,13 @@ def bench_precision_sweep(args, geneig, torch, dev):
                    "plain": {"solves_per_s": batch / tp, "ms_per_step": tp * 1e3, "eig_vs_lapack": ep,
                              "device_full_batch": dp},
                    "speedup": tp / tm, "eig_pairs_checked": int(ok.sum()),
-                   "eig_reference": "LAPACK zhegv" if have_lapack else "oracle"}
+                   "eig_reference": "LAPACK zhegv" if have_lapack else "oracle",
+                   "reference_vs_long_double": ref_ld}
             rows.append(row)
-            log("%s n=%d cond %.0e: plain %.0f/s, precision %.0f/s (x%.2f), fallback %d, eig %.2e / %.2e"
+            log("%s n=%d cond %.0e: plain %.0f/s, precision %.0f/s (x%.2f), fallback %d, eig vs LAPACK %.2e / %.2e, "
+                "vs long double %.2e / %.2e (LAPACK's own %.2e)"
                 % (fam, n, cond, batch / tp, batch / tm, tp / tm, fb, ep["max_eig_rel_err_floor1"],
-                   em["max_eig_rel_err_floor1"]))
+                   em["max_eig_rel_err_floor1"], ep["vs_long_double"], em["vs_long_double"], ref_ld))
             del W, U, info, Wp
             if fam == "generic":
                 del A, B
