@@ -48,6 +48,11 @@ int geneig_mixed_n54_batched_launch(const hipDoubleComplex* d_H, const hipDouble
  * it runs the precision form with the default controls; this launch exposes
  * them.  Destroy with geneig_solver_destroy. */
 int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int cache_mode);
+/* The same with the threads per pair of its Cholesky, K1 and K3 kernels
+ * chosen: 64 (one wave per pair) or 256 (four waves per pair, the matrix
+ * spread over the workgroup's registers); 0 picks 64 up to n = 64 and 256
+ * above.  No reference counterpart (the reference has no precision form). */
+int geneig_mixed_solver_create_bdx(geneig_solver_t* out, int n, int device_id, int block_dim_x, int cache_mode);
 int geneig_mixed_solver_launch(geneig_solver_t s, const void* d_H, const void* d_S, double* d_W, void* d_U,
                                int* d_info, int batch_size, int max_iter, double tol, int* d_iters,
                                int* h_fallback, hipStream_t stream);

@@ -120,6 +120,10 @@ int geneig_rtc_compile(int n, int bpb, int variant, const char* arch, void* buf,
  * default), as the autotuner's candidates are built. */
 int geneig_rtc_compile_rows(int n, int bpb, int variant, int k1_rows, const char* arch, void* buf, size_t cap,
                             size_t* size);
+/* The same with the threads per pair of K1 / K3 (block_dim_x 64: one wave per
+ * pair; 256: one pair per workgroup of four waves, bpb must be 1). */
+int geneig_rtc_compile_bdx(int n, int bpb, int variant, int k1_rows, int block_dim_x, const char* arch, void* buf,
+                           size_t cap, size_t* size);
 uint64_t geneig_source_hash(void);
 uint64_t geneig_fnv1a64(const void* data, size_t len, uint64_t seed);
 

@@ -61,6 +61,7 @@ def lib():
         "geneig_mixed_default_max_iter": (I, []),
         "geneig_mixed_default_tol": (D, []),
         "geneig_mixed_solver_create": (I, [P, I, I, I]),
+        "geneig_mixed_solver_create_bdx": (I, [P, I, I, I, I]),
         "geneig_spinwave_params_launch": (I, [U64, I, I, I, U64, I, D, P, P]),
         "geneig_learn_num_features": (I, []),
         "geneig_learn_hidden": (I, []),
@@ -110,6 +111,7 @@ def lib():
         "geneig_cache_path": (C.c_size_t, [I, I, C.c_char_p, I, I, C.c_char_p, C.c_size_t]),
         "geneig_rtc_compile": (I, [I, I, I, C.c_char_p, P, C.c_size_t, P]),
         "geneig_rtc_compile_rows": (I, [I, I, I, I, C.c_char_p, P, C.c_size_t, P]),
+        "geneig_rtc_compile_bdx": (I, [I, I, I, I, I, C.c_char_p, P, C.c_size_t, P]),
         "geneig_source_hash": (U64, []),
         "geneig_fnv1a64": (U64, [P, C.c_size_t, U64]),
         "geneig_make_streams": (I, [I, P]),
@@ -436,10 +438,13 @@ class GeneigMixedSolver(GeneigSolver):
     back.  ``launch`` (inherited) runs it with the default controls;
     ``launch_mixed`` exposes them and returns the number of re-solved pairs."""
 
-    def __init__(self, n, device_id=0, cache_mode=CACHE_AUTO):
+    def __init__(self, n, device_id=0, cache_mode=CACHE_AUTO, block_dim_x=0):
+        """block_dim_x: threads per pair of the Cholesky, K1 and K3 kernels (64
+        or 256; 0: 64 up to n = 64, 256 above)."""
         self._h = C.c_void_p()
-        _check(lib().geneig_mixed_solver_create(C.byref(self._h), int(n), int(device_id), int(cache_mode)),
-               "GeneigMixedSolver(n=%d)" % n)
+        _check(lib().geneig_mixed_solver_create_bdx(C.byref(self._h), int(n), int(device_id), int(block_dim_x),
+                                                    int(cache_mode)),
+               "GeneigMixedSolver(n=%d, block_dim_x=%d)" % (n, block_dim_x))
 
     def launch_mixed(self, H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None):
         batch = H.shape[0] if batch is None else batch
@@ -508,14 +513,15 @@ class GeneigMulti:
     def replays(self):
         return lib().geneig_multi_replays(self._h)
 
-def rtc_compile(n, bpb=1, variant=0, arch="gfx950", k1_rows=0):
+def rtc_compile(n, bpb=1, variant=0, arch="gfx950", k1_rows=0, block_dim_x=64):
     """Device-free hipRTC compile of the per-n kernel; returns the code object
-    bytes.  k1_rows: K1's row-chunk width (4, 8, 16; 0 = the build's default)."""
+    bytes.  k1_rows: K1's row-chunk width (4, 8, 16; 0 = the build's default);
+    block_dim_x: threads per pair of K1 / K3 (64, or 256 with bpb = 1)."""
     size = C.c_size_t(0)
-    args = (int(n), int(bpb), int(variant), int(k1_rows), arch.encode())
-    _check(lib().geneig_rtc_compile_rows(*args, None, 0, C.byref(size)), "rtc_compile(n=%d)" % n)
+    args = (int(n), int(bpb), int(variant), int(k1_rows), int(block_dim_x), arch.encode())
+    _check(lib().geneig_rtc_compile_bdx(*args, None, 0, C.byref(size)), "rtc_compile(n=%d)" % n)
     buf = C.create_string_buffer(size.value)
-    _check(lib().geneig_rtc_compile_rows(*args, buf, size.value, C.byref(size)), "rtc_compile(n=%d)" % n)
+    _check(lib().geneig_rtc_compile_bdx(*args, buf, size.value, C.byref(size)), "rtc_compile(n=%d)" % n)
     return buf.raw[: size.value]
 
 

@@ -60,6 +60,8 @@ using geneig_host::fail;
 // kListGrid* of geneig_device.inc, which the host part of this file does not
 // include).
 constexpr int kListGridK1 = 4096, kListGridK2a = 512, kListGridK2x = 1024;
+// Threads per pair of the four-wave K1 / K3 (kTileThreads of geneig_tile.inc).
+constexpr int kTileThreadsHost = 256;
 #define HIP_TRY(expr) GENEIG_HIP_TRY(expr)
 
 // ---- content hashes (FNV-1a 64, the reference's cache fingerprint,
@@ -163,12 +165,20 @@ bool write_atomic(const std::string& path, const void* data, size_t size) {
 }
 
 // ---- hipRTC ----------------------------------------------------------------
-int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vector<char>& code, int k1_rows = 0) {
+// bdx: threads per pair of K1 / K3 (64: one wave, bpb pairs per workgroup;
+// 256: the four-wave register-tiled kernels, one pair per workgroup).
+int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vector<char>& code, int k1_rows = 0,
+                int bdx = 64) {
     if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
     if (bpb < 1 || bpb > 16) return fail(GENEIG_E_ARG, "pairs per workgroup %d outside 1..16", bpb);
     if (variant != 0 && variant != 1) return fail(GENEIG_E_ARG, "variant %d is not built", variant);
     if (k1_rows != 0 && k1_rows != 4 && k1_rows != 8 && k1_rows != 16)
         return fail(GENEIG_E_ARG, "K1 row chunk %d is not 4, 8 or 16", k1_rows);
+    if (bdx != 64 && bdx != kTileThreadsHost)
+        return fail(GENEIG_E_ARG, "threads per pair %d: 64 (one wave) and %d (four waves) are built", bdx,
+                    kTileThreadsHost);
+    if (bdx == kTileThreadsHost && bpb != 1)
+        return fail(GENEIG_E_ARG, "the four-wave kernels take one pair per workgroup (bpb=%d)", bpb);
     hiprtcProgram prog = nullptr;
     if (hiprtcCreateProgram(&prog, kGeneigDeviceSource, "geneig_device.hip", 0, nullptr, nullptr) !=
         HIPRTC_SUCCESS)
@@ -177,6 +187,7 @@ int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vecto
                                      "-DGENEIG_RTC_BPB=" + std::to_string(bpb),
                                      "-DGENEIG_RTC_VARIANT=" + std::to_string(variant)};
     if (k1_rows > 0) opts.push_back("-DGENEIG_RTC_K1_RCH=" + std::to_string(k1_rows));
+    if (bdx == kTileThreadsHost) opts.push_back("-DGENEIG_RTC_TILE=1");
     for (const char* f : kRtcFlags) opts.push_back(f);
     std::vector<const char*> p;
     for (auto& s : opts) p.push_back(s.c_str());
@@ -187,8 +198,8 @@ int rtc_compile(int n, int bpb, int variant, const std::string& arch, std::vecto
         std::string log(ls + 1, '\0');
         if (ls) hiprtcGetProgramLog(prog, &log[0]);
         hiprtcDestroyProgram(&prog);
-        return fail(GENEIG_E_RTC, "hipRTC compile failed (n=%d bpb=%d variant=%d arch=%s): %s\n%s", n,
-                    bpb, variant, arch.c_str(), hiprtcGetErrorString(r), log.c_str());
+        return fail(GENEIG_E_RTC, "hipRTC compile failed (n=%d bpb=%d variant=%d bdx=%d arch=%s): %s\n%s", n,
+                    bpb, variant, bdx, arch.c_str(), hiprtcGetErrorString(r), log.c_str());
     }
     size_t cs = 0;
     hiprtcGetCodeSize(prog, &cs);
@@ -221,6 +232,7 @@ struct Module {
     hipFunction_t fr = nullptr, f1l = nullptr, f2al = nullptr, f2bl = nullptr, f2xl = nullptr, f3l = nullptr;
     hipFunction_t fz = nullptr;            // clears device counters (both variants)
     unsigned f_lds_per_pair = 0, f_k2_threads = 0, refine_lds = 0, refine_threads = 0;
+    unsigned k13_threads = 64;     // threads per pair of K1, K3 and the Cholesky stages
     unsigned f_lds_total = 0;      // larger of K1f / K3f dynamic LDS + static vectors, per pair
     unsigned long long f_ws_pair_bytes = 0;
     int n = 0, bpb = 0, variant = 0;
@@ -278,6 +290,7 @@ int load_module(const std::vector<char>& code, int n, int bpb, int variant,
     if (!rc) rc = read_global("geneig_k3_lds_bytes", &m->k3_lds, sizeof(unsigned));
     if (!rc) rc = read_global("geneig_ws_bytes_per_pair", &m->ws_pair_bytes, sizeof(unsigned long long));
     if (!rc) rc = read_global("geneig_chol64_lds_bytes", &m->chol64_lds, sizeof(unsigned));
+    if (!rc) rc = read_global("geneig_k13_threads", &m->k13_threads, sizeof(unsigned));
     if (variant == 1) {
         if (!rc) rc = read_global("geneig_f_lds_bytes_per_pair", &m->f_lds_per_pair, sizeof(unsigned));
         if (!rc) rc = read_global("geneig_f_lds_total_per_pair", &m->f_lds_total, sizeof(unsigned));
@@ -306,13 +319,13 @@ std::mutex g_module_mu;
 std::map<Key, std::weak_ptr<Module>> g_modules;
 
 struct TuneWinner {
-    int bpb = 0, variant = 0, k1_rows = 0;
+    int bpb = 0, variant = 0, k1_rows = 0, bdx = 64;
     double pairs_per_s = 0.0;
 };
 // One autotune candidate: pairs per workgroup, module variant, K1 row chunk
-// (0: the build's default).
+// (0: the build's default), threads per pair of K1 / K3.
 struct TuneCand {
-    int bpb, variant, k1_rows;
+    int bpb, variant, k1_rows, bdx;
 };
 std::mutex g_tune_mu;
 std::map<std::pair<int, std::string>, TuneWinner> g_tune;
@@ -373,6 +386,7 @@ struct geneig_solver {
     std::shared_ptr<Module> probe;
     unsigned shared_bytes = 0;
     int autotuned_bdx = 0;
+    int bdx = 64;                  // threads per pair of K1 / K3 (64 or 256)
     int k1_rows = 0;               // K1 row-chunk width of the module (0: the build's default)
     bool was_autotuned = false, sweep_ran = false;
     const char* layer = "compile";
@@ -414,8 +428,8 @@ struct geneig_solver {
 
     // Module acquisition through the cache layers (nvrtc_solver.cpp:522-597).
     int acquire(int want_bpb, int want_variant, int cache_mode, std::shared_ptr<Module>& out,
-                const char** layer_out, int k1_rows = 0) {
-        const Key key{n, want_bpb, want_variant, 64, arch, k1_rows};
+                const char** layer_out, int k1_rows = 0, int bdx = 64) {
+        const Key key{n, want_bpb, want_variant, bdx, arch, k1_rows};
         if (cache_mode != GENEIG_CACHE_NOCACHE) {
             std::lock_guard<std::mutex> lk(g_module_mu);
             auto it = g_modules.find(key);
@@ -428,7 +442,7 @@ struct geneig_solver {
                 g_modules.erase(it);
             }
         }
-        const std::string path = cache_path_str(n, want_bpb, arch, 64, want_variant, k1_rows);
+        const std::string path = cache_path_str(n, want_bpb, arch, bdx, want_variant, k1_rows);
         if (cache_mode == GENEIG_CACHE_AUTO) {
             std::ifstream f(path, std::ios::binary | std::ios::ate);
             if (f) {
@@ -453,7 +467,7 @@ struct geneig_solver {
             }
         }
         std::vector<char> code;
-        int rc = rtc_compile(n, want_bpb, want_variant, arch, code, k1_rows);
+        int rc = rtc_compile(n, want_bpb, want_variant, arch, code, k1_rows, bdx);
         if (rc) return rc;
         std::shared_ptr<Module> h;
         rc = load_module(code, n, want_bpb, want_variant, h);
@@ -523,12 +537,15 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, az, nullptr));
             if (inplace) {
                 void* ac[] = {&sp, &w, &u, &inf, &P};
-                HIP_TRY(hipModuleLaunchKernel(m.fci, (unsigned)P, 1, 1, 64, 1, 1, m.chol64_lds, stream, ac, nullptr));
+                HIP_TRY(hipModuleLaunchKernel(m.fci, (unsigned)P, 1, 1, m.k13_threads, 1, 1, m.chol64_lds, stream, ac,
+                                              nullptr));
             }
             void* a1[] = {&h, &sp, &w, &u, &inf, &ws, &P};
+            // one wave per pair, b pairs per workgroup; or one pair per
+            // workgroup of four waves (k13_threads = 256, b = 1)
             const unsigned g13 = (unsigned)((P + b - 1) / b);
-            HIP_TRY(hipModuleLaunchKernel(inplace ? m.f1i : m.fn, g13, 1, 1, 64, (unsigned)b, 1, bytes, stream, a1,
-                                          nullptr));
+            const unsigned bx = m.k13_threads == 64 ? 64u : m.k13_threads, by = m.k13_threads == 64 ? (unsigned)b : 1u;
+            HIP_TRY(hipModuleLaunchKernel(inplace ? m.f1i : m.fn, g13, 1, 1, bx, by, 1, bytes, stream, a1, nullptr));
             int cap = geneig_host::ql_log_entries();
             void* a2a[] = {&inf, &ws, &P, &cap, &ovf};
             HIP_TRY(hipModuleLaunchKernel(m.fn2a, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a,
@@ -539,8 +556,8 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.fn2x, (unsigned)std::min(P, kListGridK2x), 1, 1, m.k2_threads, 1, 1, 0,
                                           stream, a2x, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &ws, &P};
-            HIP_TRY(hipModuleLaunchKernel(inplace ? m.f3i : m.fn3, g13, 1, 1, 64, (unsigned)b, 1,
-                                          m.k3_lds * (unsigned)b, stream, a3, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(inplace ? m.f3i : m.fn3, g13, 1, 1, bx, by, 1, m.k3_lds * (unsigned)b, stream,
+                                          a3, nullptr));
         }
         return GENEIG_E_OK;
     }
@@ -567,7 +584,8 @@ struct geneig_solver {
         void* az[] = {&fb_count, &one};
         HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, az, nullptr));
         void* ac[] = {&S, &W, &U, &info, &batch};
-        HIP_TRY(hipModuleLaunchKernel(m.fc, (unsigned)batch, 1, 1, 64, 1, 1, m.chol64_lds, stream, ac, nullptr));
+        const unsigned tx = m.k13_threads;
+        HIP_TRY(hipModuleLaunchKernel(m.fc, (unsigned)batch, 1, 1, tx, 1, 1, m.chol64_lds, stream, ac, nullptr));
         for (int off = 0; off < batch; off += L.chunk_f) {
             int P = std::min(L.chunk_f, batch - off);
             const void* h = static_cast<const char*>(H) + mat * off;
@@ -579,7 +597,7 @@ struct geneig_solver {
             void* azo[] = {&ovf, &one};
             HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, azo, nullptr));
             void* a1[] = {&h, &sp, &w, &u, &inf, &solver, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.f1f, (unsigned)P, 1, 1, 64, 1, 1, m.f_lds_per_pair, stream, a1, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f1f, (unsigned)P, 1, 1, tx, 1, 1, m.f_lds_per_pair, stream, a1, nullptr));
             int cap = geneig_host::ql_log_entries();
             void* a2a[] = {&inf, &solver, &P, &cap, &ovf};
             HIP_TRY(hipModuleLaunchKernel(m.f2af, (unsigned)((P + 63) / 64), 1, 1, 64, 1, 1, m.f_k2a_lds, stream, a2a,
@@ -590,7 +608,7 @@ struct geneig_solver {
             HIP_TRY(hipModuleLaunchKernel(m.f2xf, (unsigned)std::min(P, kListGridK2x), 1, 1, m.f_k2_threads, 1, 1, 0,
                                           stream, a2x, nullptr));
             void* a3[] = {&sp, &w, &u, &inf, &solver, &P};
-            HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, 64, 1, 1, m.f_k3_lds, stream, a3, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f3f, (unsigned)P, 1, 1, tx, 1, 1, m.f_k3_lds, stream, a3, nullptr));
         }
         void* ar[] = {&H, &S, &W, &U, &info, &fb_list, &fb_count, &iters, &batch, &maxit, &tol};
         HIP_TRY(hipModuleLaunchKernel(m.fr, (unsigned)batch, 1, 1, m.refine_threads, 1, 1, m.refine_lds, stream, ar,
@@ -605,7 +623,7 @@ struct geneig_solver {
             void* azf[] = {&fovf, &one};
             HIP_TRY(hipModuleLaunchKernel(m.fz, 1, 1, 1, 64, 1, 1, 0, stream, azf, nullptr));
             void* a1[] = {&H, &S, &W, &U, &info, &solver, &cd, &fb_list, &fb_count, &off};
-            HIP_TRY(hipModuleLaunchKernel(m.f1l, g1, 1, 1, 64, 1, 1, m.lds_per_pair, stream, a1, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f1l, g1, 1, 1, tx, 1, 1, m.lds_per_pair, stream, a1, nullptr));
             int cap = geneig_host::ql_log_entries();
             void* a2a[] = {&info, &solver, &cd, &fb_list, &fb_count, &off, &cap, &fovf};
             HIP_TRY(hipModuleLaunchKernel(m.f2al, g2a, 1, 1, 64, 1, 1, m.k2a_lds, stream, a2a, nullptr));
@@ -614,7 +632,7 @@ struct geneig_solver {
             void* a2x[] = {&info, &solver, &cd, &fb_list, &off, &fovf};
             HIP_TRY(hipModuleLaunchKernel(m.f2xl, g2x, 1, 1, m.k2_threads, 1, 1, 0, stream, a2x, nullptr));
             void* a3[] = {&S, &W, &U, &info, &solver, &cd, &fb_list, &fb_count, &off};
-            HIP_TRY(hipModuleLaunchKernel(m.f3l, g1, 1, 1, 64, 1, 1, m.k3_lds, stream, a3, nullptr));
+            HIP_TRY(hipModuleLaunchKernel(m.f3l, g1, 1, 1, tx, 1, 1, m.k3_lds, stream, a3, nullptr));
         }
         if (h_fallback) {
             int count = 0;
@@ -687,11 +705,12 @@ int geneig_solver::autotune(int cache_mode, const std::vector<TuneCand>& cands, 
     (void)hipEventCreate(&e1);
     bool have = false;
     for (size_t ci = 0; ci < cands.size(); ++ci) {
-        const int b = cands[ci].bpb, v = cands[ci].variant, rows = cands[ci].k1_rows;
+        const int b = cands[ci].bpb, v = cands[ci].variant, rows = cands[ci].k1_rows, bx = cands[ci].bdx;
         std::shared_ptr<Module> m;
         const char* lay = nullptr;
-        if (acquire(b, v, cache_mode, m, &lay, rows) != GENEIG_E_OK) {
-            std::fprintf(stderr, "[autotune] bpb=%d variant=%d k1_rows=%d: rejected at compile/load\n", b, v, rows);
+        if (acquire(b, v, cache_mode, m, &lay, rows, bx) != GENEIG_E_OK) {
+            std::fprintf(stderr, "[autotune] bpb=%d variant=%d k1_rows=%d bdx=%d: rejected at compile/load\n", b, v,
+                         rows, bx);
             continue;
         }
         const unsigned bytes = m->lds_total * (unsigned)b;
@@ -745,12 +764,13 @@ int geneig_solver::autotune(int cache_mode, const std::vector<TuneCand>& cands, 
             ok = eig < tol && phase < tol;
         }
         std::fprintf(stderr,
-                     "[autotune] n=%d bpb=%d variant=%d k1_rows=%d: median %.0f pairs/s eig=%.2e phase=%.2e %s\n",
-                     n, b, v, rows, mps, eig, phase, ok ? "[OK]" : "[REJECT correctness]");
+                     "[autotune] n=%d bpb=%d variant=%d k1_rows=%d bdx=%d: median %.0f pairs/s eig=%.2e phase=%.2e %s\n",
+                     n, b, v, rows, bx, mps, eig, phase, ok ? "[OK]" : "[REJECT correctness]");
         if (ok && mps > win.pairs_per_s) {
             win.bpb = b;
             win.variant = v;
             win.k1_rows = rows;
+            win.bdx = bx;
             win.pairs_per_s = mps;
         }
     }
@@ -769,9 +789,12 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
     if (!out) return fail(GENEIG_E_ARG, "null output handle");
     *out = nullptr;
     if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
-    if (force_block_dim_x > 0 && force_block_dim_x != 64)
-        return fail(GENEIG_E_FIT, "block_dim_x=%d: only one wavefront (64 threads) per pair is built",
-                    force_block_dim_x);
+    if (force_block_dim_x > 0 && force_block_dim_x != 64 && force_block_dim_x != kTileThreadsHost)
+        return fail(GENEIG_E_FIT, "block_dim_x=%d: 64 (one wave per pair) and %d (four waves per pair) are built",
+                    force_block_dim_x, kTileThreadsHost);
+    if (force_block_dim_x == kTileThreadsHost && bpb_request > 1)
+        return fail(GENEIG_E_FIT, "block_dim_x=%d takes one pair per workgroup, not %d", force_block_dim_x,
+                    bpb_request);
     if (bpb_request > 16) return fail(GENEIG_E_ARG, "pairs per workgroup %d > 16", bpb_request);
     if (cache_mode < 0 || cache_mode > 2) return fail(GENEIG_E_ARG, "bad cache mode %d", cache_mode);
     std::unique_ptr<geneig_solver> s(new geneig_solver);
@@ -798,6 +821,10 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
         if (rank > deepest) deepest = rank;
     };
     std::shared_ptr<Module> keep;   // probe module, kept loaded for the final acquisition
+    // Threads per pair of K1 / K3 unless the caller or the autotuner picks:
+    // four waves above n = 64 (the one-wave kernels keep the matrix in LDS
+    // there, and do not fit it from n = 100), one wave up to it.
+    int bdx = force_block_dim_x > 0 ? force_block_dim_x : (n > 64 ? kTileThreadsHost : 64);
     if (tuning_mode == GENEIG_TUNING_AUTOTUNE && force_block_dim_x <= 0) {
         TuneWinner win;
         bool hit = false;
@@ -808,34 +835,42 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
         }
         if (!hit && cache_mode == GENEIG_CACHE_AUTO) {
             std::ifstream f(tune_path_str(n, s->arch));
-            if (f >> win.bpb >> win.variant >> win.pairs_per_s >> win.k1_rows && win.bpb >= 1 && win.bpb <= 16 &&
-                (win.k1_rows == 0 || win.k1_rows == 4 || win.k1_rows == 8 || win.k1_rows == 16))
+            if (f >> win.bpb >> win.variant >> win.pairs_per_s >> win.k1_rows >> win.bdx && win.bpb >= 1 &&
+                win.bpb <= 16 && (win.k1_rows == 0 || win.k1_rows == 4 || win.k1_rows == 8 || win.k1_rows == 16) &&
+                (win.bdx == 64 || (win.bdx == kTileThreadsHost && win.bpb == 1)))
                 hit = true;
             else
                 win = TuneWinner();
         }
         if (bpb_request > 0 && hit && win.bpb != bpb_request) hit = false;
         if (!hit) {
-            // probe the per-pair LDS need with a bpb=1 module, then sweep
-            std::shared_ptr<Module> m1;
-            const char* lay = nullptr;
-            int rc = s->acquire(1, 0, cache_mode, m1, &lay);
-            if (rc) return rc;
-            note(lay);
-            keep = m1;
-            // stage 1: pairs per workgroup with the default tile; stage 2: the
-            // K1 row-chunk width (the tile shape of its register walks) at the
-            // winning pairs per workgroup, the stage-1 winner first as the
-            // correctness reference (n <= 64: the walks exist only there)
+            // stage 1: threads per pair (the reference's knob) and, for one
+            // wave per pair, pairs per workgroup, the one-wave candidates
+            // sized by a probe bpb = 1 module (n < 100: above, their LDS
+            // matrix does not fit); stage 2: the K1 row-chunk width (the
+            // tile shape of its register walks) at the winning pairs per
+            // workgroup, the stage-1 winner first as the correctness
+            // reference (one wave per pair, n <= 64: the walks exist only
+            // there)
             std::vector<TuneCand> cands;
-            for (int b : {1, 2, 4, 8})
-                if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_total)) cands.push_back({b, 0, 0});
+            if (n < 100) {
+                std::shared_ptr<Module> m1;
+                const char* lay = nullptr;
+                int rc = s->acquire(1, 0, cache_mode, m1, &lay);
+                if (rc) return rc;
+                note(lay);
+                keep = m1;
+                for (int b : {1, 2, 4, 8})
+                    if ((bpb_request <= 0 || b == bpb_request) && s->fits(b, m1->lds_total))
+                        cands.push_back({b, 0, 0, 64});
+            }
+            if (bpb_request <= 1) cands.push_back({1, 0, 0, kTileThreadsHost});
             if (cands.empty()) return fail(GENEIG_E_FIT, "n=%d: no pairs-per-workgroup candidate fits", n);
-            rc = s->autotune(cache_mode, cands, win);
+            int rc = s->autotune(cache_mode, cands, win);
             if (rc) return rc;
-            if (n <= 64) {
+            if (n <= 64 && win.bdx == 64) {
                 const TuneWinner first = win;
-                std::vector<TuneCand> tiles = {{first.bpb, 0, 0}, {first.bpb, 0, 4}, {first.bpb, 0, 16}};
+                std::vector<TuneCand> tiles = {{first.bpb, 0, 0, 64}, {first.bpb, 0, 4, 64}, {first.bpb, 0, 16, 64}};
                 win = TuneWinner();
                 rc = s->autotune(cache_mode, tiles, win);
                 if (rc) return rc;
@@ -844,8 +879,8 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
             g_tune[{n, s->arch}] = win;
             if (cache_mode == GENEIG_CACHE_AUTO) {
                 char line[128];
-                int len = std::snprintf(line, sizeof line, "%d %d %.1f %d\n", win.bpb, win.variant, win.pairs_per_s,
-                                        win.k1_rows);
+                int len = std::snprintf(line, sizeof line, "%d %d %.1f %d %d\n", win.bpb, win.variant,
+                                        win.pairs_per_s, win.k1_rows, win.bdx);
                 write_atomic(tune_path_str(n, s->arch), line, (size_t)len);
             }
         }
@@ -853,8 +888,10 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
         variant = win.variant;
         s->k1_rows = win.k1_rows;
         s->was_autotuned = true;
-        s->autotuned_bdx = 64;
+        s->autotuned_bdx = win.bdx;
+        bdx = win.bdx;
     }
+    if (bpb <= 0 && bdx == kTileThreadsHost) bpb = 1;
     if (bpb <= 0) {
         std::shared_ptr<Module> m1;
         const char* lay = nullptr;
@@ -868,10 +905,11 @@ int geneig_solver_create(geneig_solver_t* out, int n, int device_id, int bpb_req
                         m1->lds_total, s->lds_per_block);
     }
     const char* lay = "compile";
-    int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay, s->k1_rows);
+    int rc = s->acquire(bpb, variant, cache_mode, s->module, &lay, s->k1_rows, bdx);
     if (rc) return rc;
     note(lay);
     s->probe = std::move(keep);
+    s->bdx = bdx;
     s->acquire_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     s->layer = deepest == 2 ? "compile" : deepest == 1 ? "disk" : "in-process";
     s->bpb = bpb;
@@ -905,9 +943,17 @@ int geneig_solver_launch_inplace(geneig_solver_t s, void* d_H, void* d_S, double
 }
 
 int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int cache_mode) {
+    return geneig_mixed_solver_create_bdx(out, n, device_id, 0, cache_mode);
+}
+
+int geneig_mixed_solver_create_bdx(geneig_solver_t* out, int n, int device_id, int block_dim_x, int cache_mode) {
     if (!out) return fail(GENEIG_E_ARG, "null output handle");
     *out = nullptr;
     if (n < 1 || n > 128) return fail(GENEIG_E_ARG, "n=%d outside the supported range 1..128", n);
+    if (block_dim_x > 0 && block_dim_x != 64 && block_dim_x != kTileThreadsHost)
+        return fail(GENEIG_E_FIT, "block_dim_x=%d: 64 (one wave per pair) and %d (four waves per pair) are built",
+                    block_dim_x, kTileThreadsHost);
+    const int bdx = block_dim_x > 0 ? block_dim_x : (n > 64 ? kTileThreadsHost : 64);
     if (cache_mode < 0 || cache_mode > 2) return fail(GENEIG_E_ARG, "bad cache mode %d", cache_mode);
     std::unique_ptr<geneig_solver> s(new geneig_solver);
     s->n = n;
@@ -921,12 +967,13 @@ int geneig_mixed_solver_create(geneig_solver_t* out, int n, int device_id, int c
     s->lds_per_block = (size_t)v;
     const auto t0 = std::chrono::steady_clock::now();
     const char* lay = "compile";
-    const int rc = s->acquire(1, 1, cache_mode, s->module, &lay);
+    const int rc = s->acquire(1, 1, cache_mode, s->module, &lay, 0, bdx);
     if (rc) return rc;
     s->acquire_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     s->layer = lay;
     s->bpb = 1;
     s->variant = 1;
+    s->bdx = bdx;
     const Module& m = *s->module;
     // every kernel of the launch must fit, static broadcast vectors included:
     // the FP64 ones re-solve the pairs the refinement hands back
@@ -1016,7 +1063,7 @@ size_t geneig_solver_workspace_bytes(geneig_solver_t s, int batch_size) {
 int geneig_solver_matrix_size(geneig_solver_t s) { return s ? s->n : 0; }
 int geneig_solver_batches_per_block(geneig_solver_t s) { return s ? s->bpb : 0; }
 void geneig_solver_block_dim(geneig_solver_t s, unsigned* x, unsigned* y, unsigned* z) {
-    if (x) *x = s ? 64u : 0u;
+    if (x) *x = s ? (unsigned)s->bdx : 0u;
     if (y) *y = s ? (unsigned)s->bpb : 0u;
     if (z) *z = s ? 1u : 0u;
 }
@@ -1028,7 +1075,9 @@ const char* geneig_solver_cache_layer_used(geneig_solver_t s) { return s ? s->la
 double geneig_solver_acquire_module_ms(geneig_solver_t s) { return s ? s->acquire_ms : 0.0; }
 int geneig_solver_variant(geneig_solver_t s) { return s ? s->variant : -1; }
 int geneig_solver_k1_chunk_rows(geneig_solver_t s) {
-    if (!s || s->n > 64) return 0;     // above n = 64 K1 works in LDS, without register walks
+    // above n = 64 the one-wave K1 works in LDS, and the four-wave K1 has no
+    // register walks: neither has a row chunk
+    if (!s || s->n > 64 || s->bdx != 64) return 0;
     if (s->k1_rows > 0) return s->k1_rows;
     return s->variant == 1 ? 16 : 8;   // the defaults of k1_rch<T>(): FP64 walks 8 rows, complex64 16
 }
@@ -1052,8 +1101,13 @@ int geneig_rtc_compile(int n, int bpb, int variant, const char* arch, void* buf,
 
 int geneig_rtc_compile_rows(int n, int bpb, int variant, int k1_rows, const char* arch, void* buf, size_t cap,
                             size_t* size) {
+    return geneig_rtc_compile_bdx(n, bpb, variant, k1_rows, 64, arch, buf, cap, size);
+}
+
+int geneig_rtc_compile_bdx(int n, int bpb, int variant, int k1_rows, int block_dim_x, const char* arch, void* buf,
+                           size_t cap, size_t* size) {
     std::vector<char> code;
-    int rc = rtc_compile(n, bpb, variant, arch ? arch : "gfx950", code, k1_rows);
+    int rc = rtc_compile(n, bpb, variant, arch ? arch : "gfx950", code, k1_rows, block_dim_x);
     if (rc) return rc;
     if (size) *size = code.size();
     if (buf && cap >= code.size()) std::memcpy(buf, code.data(), code.size());

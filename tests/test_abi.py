@@ -78,6 +78,17 @@ def test_rtc_k1_row_chunks_compile_without_device(geneig, n, rows):
     assert code != geneig.rtc_compile(n, 1)
 
 
+@pytest.mark.parametrize("n,variant", [(1, 0), (8, 0), (54, 0), (54, 1), (96, 1), (100, 0), (128, 0), (128, 1)])
+def test_rtc_four_wave_modules_compile_without_device(geneig, n, variant):
+    """Both module variants with K1 / K3 (and the Cholesky stages) on one
+    workgroup of four waves per pair, n up to 128, and distinct from the
+    one-wave module."""
+    code = geneig.rtc_compile(n, 1, variant=variant, block_dim_x=256)
+    assert code[:4] == b"\x7fELF" and len(code) > 1024
+    if n <= 64:
+        assert code != geneig.rtc_compile(n, 1, variant=variant)
+
+
 def test_rtc_rejects_bad_arguments(geneig):
     with pytest.raises(geneig.GeneigError):
         geneig.rtc_compile(0)
@@ -87,6 +98,10 @@ def test_rtc_rejects_bad_arguments(geneig):
         geneig.rtc_compile(32, 1, variant=7)
     with pytest.raises(geneig.GeneigError, match="row chunk"):
         geneig.rtc_compile(32, 1, k1_rows=32)        # would read past the vectors' zero pad
+    with pytest.raises(geneig.GeneigError, match="one pair per workgroup"):
+        geneig.rtc_compile(32, 2, block_dim_x=256)
+    with pytest.raises(geneig.GeneigError, match="threads per pair"):
+        geneig.rtc_compile(32, 1, block_dim_x=128)
 
 
 def test_cache_dir_resolution(geneig, monkeypatch):
@@ -110,7 +125,7 @@ def test_cache_path_key(geneig, monkeypatch):
 
 def test_source_hash_is_fnv1a_of_device_source(geneig, oracle):
     csrc = os.path.join(ROOT, "sunny-batched-geneig_amd", "csrc")
-    src = b"".join(open(os.path.join(csrc, f), "rb").read() for f in ("geneig_device.inc", "geneig_refine.inc"))
+    src = b"".join(open(os.path.join(csrc, f), "rb").read() for f in ("geneig_device.inc", "geneig_tile.inc", "geneig_rtc0.inc", "geneig_refine.inc"))
     assert geneig.lib().geneig_source_hash() == oracle.fnv1a64(src)
     buf = C.create_string_buffer(b"foobar", 6)
     assert geneig.lib().geneig_fnv1a64(C.cast(buf, C.c_void_p), 6, 0) == 0x85944171f73967e8
@@ -119,8 +134,10 @@ def test_source_hash_is_fnv1a_of_device_source(geneig, oracle):
 def test_solver_create_reports_errors(geneig):
     with pytest.raises(geneig.GeneigError, match="outside the supported range"):
         geneig.GeneigSolver(0)
-    with pytest.raises(geneig.GeneigError, match="64 threads"):
+    with pytest.raises(geneig.GeneigError, match="one wave per pair"):
         geneig.GeneigSolver(54, force_block_dim_x=128)
+    with pytest.raises(geneig.GeneigError, match="one pair per workgroup"):
+        geneig.GeneigSolver(54, batches_per_block_request=2, force_block_dim_x=256)
 
 
 def test_product_does_not_touch_oracle():

@@ -100,23 +100,29 @@ def test_cache_invalidation_recompiles(geneig, device, cache_dir):
 
 def test_autotune_then_cache_hit(geneig, oracle, device, cache_dir):
     """Sweep on first construction, winner reused after (test_autotune_n54.cpp:308-346).
-    The sweep picks pairs per workgroup, then K1's row-chunk width (the tile
-    shape of its register walks) at that pairs per workgroup; both are
-    persisted and reused."""
+    The sweep picks threads per pair (64: one wave, swept over pairs per
+    workgroup; 256: four waves, one pair per workgroup), then, for one wave
+    per pair, K1's row-chunk width (the tile shape of its register walks) at
+    the winning pairs per workgroup; all are persisted and reused."""
     s1 = geneig.GeneigSolver(32, tuning_mode=geneig.TUNING_AUTOTUNE)
     assert s1.was_autotuned() and s1.autotune_sweep_ran()
-    assert s1.batches_per_block() in (1, 2, 4, 8)
-    assert s1.k1_chunk_rows() in (4, 8, 16)
+    bdx = s1.autotuned_block_dim_x()
+    assert bdx in (64, 256) and s1.block_dim()[0] == bdx
+    assert s1.batches_per_block() in ((1, 2, 4, 8) if bdx == 64 else (1,))
+    assert s1.k1_chunk_rows() in ((4, 8, 16) if bdx == 64 else (0,))
     s2 = geneig.GeneigSolver(32, tuning_mode=geneig.TUNING_AUTOTUNE)
     assert s2.was_autotuned() and not s2.autotune_sweep_ran()
     assert s2.batches_per_block() == s1.batches_per_block()
     assert s2.k1_chunk_rows() == s1.k1_chunk_rows()
+    assert s2.block_dim() == s1.block_dim()
     tune = [f for f in os.listdir(cache_dir) if f.startswith("autotune_32_")]
     assert len(tune) == 1
     fields = open(os.path.join(cache_dir, tune[0])).read().split()
-    print("autotune record (bpb variant pairs/s k1_rows):", fields, flush=True)
-    assert len(fields) == 4 and int(fields[0]) == s1.batches_per_block()
-    assert int(fields[3]) in (0, 4, 16) and (int(fields[3]) or 8) == s1.k1_chunk_rows()
+    print("autotune record (bpb variant pairs/s k1_rows block_dim_x):", fields, flush=True)
+    assert len(fields) == 5 and int(fields[0]) == s1.batches_per_block() and int(fields[4]) == bdx
+    assert int(fields[3]) in (0, 4, 16)
+    if bdx == 64:
+        assert (int(fields[3]) or 8) == s1.k1_chunk_rows()
     A, B = oracle.gen_generic(32, 16, seed=3)
     W, X, info = run(geneig, s2, A, B, device)
     Wo, Xo, _ = oracle.zhegv(A, B)
@@ -237,14 +243,15 @@ print("ok", m["eig_rel_scaled"])
 
 
 def test_largest_size_solves_and_larger_sizes_are_refused(geneig, oracle, device, cache_dir):
-    """Above n = 64 every stage keeps the pair's n x LD working matrix in LDS
-    (16 n LD bytes in FP64, LD = n + 1 for even n, plus the two broadcast
-    vectors): n = 98 needs 159,840 B and fits the 160 KB a workgroup may use,
-    n = 100 needs 166,208 B and does not.  The reference's multi-N test goes to
-    n = 128 (tests/test_nvrtc_multiN.cpp:218-271); here n = 98 solves to the
-    FP64 bars in both forms, and n = 100 and 128 are refused at construction
-    with GENEIG_E_FIT (code 4) for both forms (the precision form keeps the FP64
-    kernels for the pairs its refinement hands back)."""
+    """The one-wave kernels keep the pair's n x LD working matrix in LDS above
+    n = 64 (16 n LD bytes in FP64, LD = n + 1 for even n, plus the two
+    broadcast vectors): n = 98 needs 159,840 B and fits the 160 KB a
+    workgroup may use, n = 100 needs 166,208 B and does not, so a solver
+    forced to one wave per pair is refused there with GENEIG_E_FIT (code 4).
+    The four-wave kernels (the default above n = 64) hold the matrix in
+    registers and reach the reference's n = 128 (tests/test_nvrtc_multiN.cpp:
+    218-271).  The precision form's refinement keeps X (n x LD complex) in LDS,
+    so it stops at n = 98 too."""
     from test_gpu_parity import check_pairs
 
     n = 98
@@ -252,14 +259,22 @@ def test_largest_size_solves_and_larger_sizes_are_refused(geneig, oracle, device
     Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
     Wo, Xo, io = oracle.zhegv(Ah, Bh)
     assert (io == 0).all()
-    for tag, s in (("fp64", geneig.GeneigSolver(n)), ("precision", geneig.GeneigMixedSolver(n))):
+    for tag, s in (("fp64 one wave", geneig.GeneigSolver(n, force_block_dim_x=64)),
+                   ("precision", geneig.GeneigMixedSolver(n))):
         print("n=%d %s: %d B LDS per workgroup" % (n, tag, s.shared_mem_bytes()), flush=True)
         W, X, info = run(geneig, s, A, B, device)
         assert (info == 0).all()
         check_pairs(oracle, Ah, Bh, W, X, Wo, Xo, n, "n=%d %s" % (n, tag))
     for big in (100, 128):
-        for make in (geneig.GeneigSolver, geneig.GeneigMixedSolver):
+        Ah, Bh = oracle.gen_generic(big, 4, seed=big)
+        Wo, Xo, io = oracle.zhegv(Ah, Bh)
+        s = geneig.GeneigSolver(big)
+        assert s.block_dim() == (256, 1, 1)
+        W, X, info = run(geneig, s, Ah, Bh, device)
+        assert (info == 0).all()
+        check_pairs(oracle, Ah, Bh, W, X, Wo, Xo, big, "n=%d fp64 four waves" % big)
+        for make in (lambda m: geneig.GeneigSolver(m, force_block_dim_x=64), geneig.GeneigMixedSolver):
             with pytest.raises(geneig.GeneigError) as e:
                 make(big)
-            print("n=%d %s refused: %s" % (big, make.__name__, e.value), flush=True)
+            print("n=%d refused: %s" % (big, e.value), flush=True)
             assert "(code 4)" in str(e.value) and "LDS" in str(e.value)
