@@ -112,8 +112,11 @@ int run_solve(int n, int batch, const char* in_path, const char* out_path) {
     check(solver.matrix_size() == n, "matrix_size");
     const int bpb = solver.batches_per_block();
     const dim3 bd = solver.block_dim();
-    check(bpb >= 1 && bd.x == 64 && bd.y == (unsigned)bpb && bd.z == 1, "block_dim",
-          std::to_string(bd.x) + "x" + std::to_string(bd.y) + "x" + std::to_string(bd.z));
+    // 64 threads per pair (bpb pairs per workgroup) up to n = 64, one pair on
+    // 256 threads above
+    check(bpb >= 1 && bd.x == (n > 64 ? 256u : 64u) && bd.y == (unsigned)bpb && bd.z == 1 &&
+              (bd.x == 64 || bpb == 1),
+          "block_dim", std::to_string(bd.x) + "x" + std::to_string(bd.y) + "x" + std::to_string(bd.z));
     check(solver.shared_mem_bytes() > 0, "shared_mem_bytes", std::to_string(solver.shared_mem_bytes()));
     check(!solver.was_autotuned() && !solver.autotune_sweep_ran(), "not_autotuned");
     const std::string layer = solver.cache_layer_used();
@@ -160,7 +163,7 @@ int run_solve(int n, int batch, const char* in_path, const char* out_path) {
         check(std::string(again.cache_layer_used()) == "in-process", "second_solver_in_process",
               again.cache_layer_used());
     }
-    const ModuleCacheKey key{n, bpb, 950, 64};
+    const ModuleCacheKey key{n, bpb, 950, (int)bd.x};
     const std::string path = NvrtcGeneigSolver::cubin_path(key);
     check(file_exists(path), "cubin_path_on_disk", path);
 

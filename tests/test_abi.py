@@ -64,7 +64,8 @@ def test_rtc_compiles_without_device(geneig, n):
 @pytest.mark.parametrize("n", [3, 16, 33, 54, 64, 96, 128])
 def test_rtc_precision_module_compiles_without_device(geneig, n):
     """The precision-form module (variant 1: FP64 Cholesky, complex64 stages,
-    refinement with split bf16 products, FP64 list kernels) for n up to 128."""
+    refinement with FP64 and FP32 matrix-core products, FP64 list kernels) for n
+    up to 128."""
     code = geneig.rtc_compile(n, 1, variant=1)
     assert code[:4] == b"\x7fELF" and len(code) > 1024
 

@@ -250,8 +250,8 @@ def test_largest_size_solves_and_larger_sizes_are_refused(geneig, oracle, device
     forced to one wave per pair is refused there with GENEIG_E_FIT (code 4).
     The four-wave kernels (the default above n = 64) hold the matrix in
     registers and reach the reference's n = 128 (tests/test_nvrtc_multiN.cpp:
-    218-271).  The precision form's refinement keeps X (n x LD complex) in LDS,
-    so it stops at n = 98 too."""
+    218-271).  The precision form's refinement keeps X (n x LD complex) in LDS:
+    n = 100 still fits (163,084 B), n = 128 is refused."""
     from test_gpu_parity import check_pairs
 
     n = 98
@@ -273,7 +273,16 @@ def test_largest_size_solves_and_larger_sizes_are_refused(geneig, oracle, device
         W, X, info = run(geneig, s, Ah, Bh, device)
         assert (info == 0).all()
         check_pairs(oracle, Ah, Bh, W, X, Wo, Xo, big, "n=%d fp64 four waves" % big)
-        for make in (lambda m: geneig.GeneigSolver(m, force_block_dim_x=64), geneig.GeneigMixedSolver):
+        refused = [lambda m: geneig.GeneigSolver(m, force_block_dim_x=64)]
+        if big == 100:      # the refinement's X (100 x 101 complex) still fits: 163,084 B
+            sm = geneig.GeneigMixedSolver(big)
+            print("n=%d precision: %d B LDS per workgroup" % (big, sm.shared_mem_bytes()), flush=True)
+            W, X, info = run(geneig, sm, Ah, Bh, device)
+            assert (info == 0).all()
+            check_pairs(oracle, Ah, Bh, W, X, Wo, Xo, big, "n=%d precision" % big)
+        else:
+            refused.append(geneig.GeneigMixedSolver)
+        for make in refused:
             with pytest.raises(geneig.GeneigError) as e:
                 make(big)
             print("n=%d refused: %s" % (big, e.value), flush=True)
