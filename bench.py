@@ -597,6 +597,54 @@ def bench_spinwave_fused(args, geneig, torch, dev):
     print(json.dumps(line), flush=True)
 
 
+def bench_spinwave_eta(args, geneig, torch, dev):
+    """The precision form for the pencil (eta, S) (geneig_mixed_eta_n54_batched_launch:
+    A = eta not read, the refinement's A X a row scaling) against the general
+    precision form and the plain FP64 drop-in on the same n = 54 spin-wave
+    pairs of the headline workload; the eta form's outputs must equal the
+    general form's bit for bit."""
+    n, batch = 54, args.batch
+    A, B = geneig.gen_spinwave(27, batch, seed=args.seed, grid=(64, 64, 32))
+    W, U, info = geneig.empty_outputs(n, batch, dev)
+    W2, U2, i2 = geneig.empty_outputs(n, batch, dev)
+    it, it2 = (torch.zeros(batch, dtype=torch.int32, device=dev) for _ in range(2))
+    stream = torch.cuda.current_stream()
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / args.steps
+
+    tg = timed(lambda: geneig.mixed_n54_batched(A, B, W, U, info, iters=it, stream=stream, report_fallback=False))
+    te = timed(lambda: geneig.mixed_eta_n54_batched(B, W2, U2, i2, iters=it2, stream=stream, report_fallback=False))
+    same = bool(torch.equal(W, W2) and torch.equal(U, U2) and torch.equal(info, i2) and torch.equal(it, it2))
+    dev_e = device_check(torch, A, B, W2, U2, i2)
+    tp = timed(lambda: geneig.full_n54_batched(A, B, W, U, info, stream=stream))
+    log("eta form %.2f ms, general precision form %.2f ms, plain FP64 %.2f ms, outputs equal: %s"
+        % (te * 1e3, tg * 1e3, tp * 1e3, same))
+    v, c = np.unique(it2.cpu().numpy(), return_counts=True)
+    line = {
+        "metric": "n=54 spin-wave solves/s, precision form for the pencil (eta, S) (A = eta not read)",
+        "value": batch / te, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": te * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 bulk + f64 (Cholesky, refinement)",
+        "data": "synthetic (device-generated spin-wave pairs, seed %d)" % args.seed,
+        "config": {"workload": "spin-wave (eta, H(q)) pairs, n=54, %d q on a 64x64x32 grid" % batch,
+                   "global_batch": batch, "parallelism": "none (1 GPU)"},
+        "outputs_equal_general_precision_form": same, "device_full_batch": dev_e,
+        "refine_steps_histogram": {str(int(a)): int(b) for a, b in zip(v, c)},
+        "general_precision_form": {"value": batch / tg, "ms_per_step": tg * 1e3},
+        "plain_fp64": {"value": batch / tp, "ms_per_step": tp * 1e3},
+        "speedup_vs_general": tg / te, "speedup_vs_plain": tp / te,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_host_chunked(args, geneig, torch, dev):
     """n = 54 spin-wave pairs held in pinned host memory: launch_chunked_host copies
     each chunk in, solves it and copies it out on its stream, so with several
@@ -664,7 +712,7 @@ def main():
                                                                   "LAPACK accuracy reference)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=["spinwave54", "mixed-sizes", "learned", "host-chunked", "generic",
-                                           "precision-sweep", "spinwave-fused"],
+                                           "precision-sweep", "spinwave-fused", "spinwave-eta"],
                     default="spinwave54",
                     help="spinwave54: the headline metric; mixed-sizes: 200,000 generic pairs over "
                          "n in {8,...,96}, one graph launch vs one launch per size; learned: train the "
@@ -719,6 +767,9 @@ def main():
         return
     if args.workload == "spinwave-fused":
         bench_spinwave_fused(args, geneig, torch, dev)
+        return
+    if args.workload == "spinwave-eta":
+        bench_spinwave_eta(args, geneig, torch, dev)
         return
     n, nsub, batch = 54, 27, args.batch
     grid = (64, 64, 32)

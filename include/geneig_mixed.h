@@ -42,7 +42,19 @@ int geneig_mixed_n54_batched_launch(const hipDoubleComplex* d_H, const hipDouble
                                     hipDoubleComplex* d_U, int* d_info, int batch_size, int max_iter,
                                     double tol, int* d_iters, int* h_fallback, hipStream_t stream);
 
-/* Precision form for any n (1..128): a runtime solver (geneig_solver.h) whose
+/* The precision form for the pencil (eta, S) of linear spin-wave theory,
+ * A = eta = diag(I_27, -I_27) (not passed, not read) and S = H(q), Hermitian
+ * positive definite, n = 54: what Sunny's hegvd_batched! is handed
+ * (benchmarks/sw08_baseline.jl:163-174).  The complex64 reduction and the FP64
+ * fallback form eta in registers, and the refinement scales X by eta instead
+ * of a matrix-core pass for A X; the arithmetic is otherwise the same, so W, U
+ * and info equal geneig_mixed_n54_batched_launch's on (eta, S).  No reference
+ * counterpart (the reference has no precision form). */
+int geneig_mixed_eta_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U,
+                                        int* d_info, int batch_size, int max_iter, double tol, int* d_iters,
+                                        int* h_fallback, hipStream_t stream);
+
+/* Precision form for any n (1..100): a runtime solver (geneig_solver.h) whose
  * hipRTC module holds the precision-form kernels specialized for n (cached
  * like the FP64 module, variant 1 in the cache key).  geneig_solver_launch on
  * it runs the precision form with the default controls; this launch exposes

@@ -12,6 +12,11 @@
 //   FB   the listed pairs run through the FP64 pipeline (K1/K2/K3 over the list,
 //        whose length the kernels read on the device: no host round trip)
 // K1f..K3f and FB are chunked to bound the workspace, exactly as the FP64 path.
+// The pencil (eta, S) of linear spin-wave theory, eta = diag(I_{n/2},
+// -I_{n/2}), has its own launcher: A is never read (K1f and the fallback K1
+// form eta in registers, the refinement scales X by eta instead of a matrix-
+// core pass for A X), with the same arithmetic, so its outputs equal the
+// general launcher's on the materialized pair.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,12 +63,14 @@ __global__ void __launch_bounds__(64, GENEIG_CHOL64_WAVES_PER_EU) chol64_kernel(
     geneig::k1_batched<kN, 1, geneig::MODE_CHOL64, double>(B, B, W, U, info, nullptr, batch, batch, smem, vecs);
 }
 
+template <bool ETA>
 __global__ void __launch_bounds__(64) k1f_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                   double* __restrict__ W, cplx* __restrict__ U,
                                                   int* __restrict__ info, unsigned char* ws, int batch) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplxf vecs[CF::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_MIXED, float>(A, B, W, U, info, ws, batch, batch, smem, vecs);
+    geneig::k1_batched<kN, 1, ETA ? geneig::MODE_SW : geneig::MODE_MIXED, float>(A, B, W, U, info, ws, batch,
+                                                                                  batch, smem, vecs);
 }
 
 // K2 as the FP64 path runs it: chases of 64 pairs per wave with the rotations
@@ -102,18 +109,20 @@ __global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3f_kernel(const 
     geneig::k3_batched<kN, 1, geneig::MODE_MIXED, float>(B, W, U, info, ws, batch, batch, smem, vecs);
 }
 
+template <bool ETA>
 __global__ void __launch_bounds__(64 * RC::NT, GENEIG_REFINE_WAVES_PER_EU) refine_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                               double* __restrict__ W, cplx* __restrict__ U,
                                                               const int* __restrict__ info, int* __restrict__ fb_list,
                                                               int* __restrict__ fb_count, int* __restrict__ iters,
                                                               int batch, int maxit, double tol, int first) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-    geneig::refine_batched<kN>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem, first);
+    geneig::refine_batched<kN, ETA>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem, first);
 }
 
 // The FP64 pipeline over the fallback list: round `off` covers entries off ..
 // off + chunk - 1 of the list, whose length the kernels read on the device
 // (dispatch_slots), so the host never waits for it.
+template <bool ETA>
 __global__ void __launch_bounds__(64) k1l_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                   double* __restrict__ W, cplx* __restrict__ U,
                                                   int* __restrict__ info, unsigned char* ws, int chunk,
@@ -121,7 +130,7 @@ __global__ void __launch_bounds__(64) k1l_kernel(const cplx* __restrict__ A, con
                                                   int off) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ cplx vecs[CD::VEC];
-    geneig::k1_batched<kN, 1, geneig::MODE_FULL, double, true>(
+    geneig::k1_batched<kN, 1, ETA ? geneig::MODE_FULL_SW : geneig::MODE_FULL, double, true>(
         A, B, W, U, info, ws,
         geneig::dispatch_slots(0, dcount, off, chunk), chunk, smem,
                                                          vecs, list + off);
@@ -184,15 +193,19 @@ hipError_t set_lds_once(const void* kernel, unsigned bytes) {
         if (_e != hipSuccess) return _e; \
     } while (0)
 
+// ETA: A = eta, not read (A may be any pointer; the kernels that would read it
+// get B in its place).
+template <bool ETA>
 hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info, int batch, int maxit,
                      double tol, int* iters, int* h_fallback, hipStream_t stream) {
+    if (ETA) A = B;
     if (h_fallback) *h_fallback = 0;
     if (batch < 0) return hipErrorInvalidValue;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel), kK1LdsF));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel<ETA>), kK1LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), kK3LdsF));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), kK1LdsD));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel<ETA>), RC::LDS));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel<ETA>), kK1LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2af_kernel), kK2aLdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2al_kernel), kK2aLdsD));
@@ -218,7 +231,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         int* ovf = geneig_host::region_ctl(w, P, WF::PAIR_BYTES);
         hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ovf, 1);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k1f_kernel, dim3((unsigned)P), dim3(64), kK1LdsF, st, A + mat * off, B + mat * off,
+        hipLaunchKernelGGL(k1f_kernel<ETA>, dim3((unsigned)P), dim3(64), kK1LdsF, st, A + mat * off, B + mat * off,
                            W + (size_t)kN * off, U + mat * off, info + off, w, P);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2af_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLdsF, st, info + off, w, P,
@@ -245,7 +258,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
             e = hipGetLastError();
             if (e == hipSuccess) e = bulk(off, P, solver + L.region_f * i, subs[i]);
             if (e == hipSuccess) {
-                hipLaunchKernelGGL(refine_kernel, dim3((unsigned)P), dim3(64 * RC::NT), RC::LDS, subs[i], A, B, W, U,
+                hipLaunchKernelGGL(refine_kernel<ETA>, dim3((unsigned)P), dim3(64 * RC::NT), RC::LDS, subs[i], A, B, W, U,
                                    info, fb_list, fb_count, iters, off + P, maxit, tol, off);
                 e = hipGetLastError();
             }
@@ -259,7 +272,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         RETURN_IF(hipGetLastError());
         for (int off = 0; off < batch; off += L.chunk_f)
             RETURN_IF(bulk(off, std::min(L.chunk_f, batch - off), solver, stream));
-        hipLaunchKernelGGL(refine_kernel, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, W, U, info,
+        hipLaunchKernelGGL(refine_kernel<ETA>, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, W, U, info,
                            fb_list, fb_count, iters, batch, maxit, tol, 0);
         RETURN_IF(hipGetLastError());
     }
@@ -279,7 +292,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         const int off = r * cd;
         hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, fovf, 1);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k1l_kernel, dim3(g1), dim3(64), kK1LdsD, stream, A, B, W, U, info, solver, cd, lst, cnt,
+        hipLaunchKernelGGL(k1l_kernel<ETA>, dim3(g1), dim3(64), kK1LdsD, stream, A, B, W, U, info, solver, cd, lst, cnt,
                            off);
         RETURN_IF(hipGetLastError());
         hipLaunchKernelGGL(k2al_kernel, dim3(g2a), dim3(64), kK2aLdsD, stream, info, solver, cd, lst, cnt, off,
@@ -317,8 +330,16 @@ extern "C" {
 int geneig_mixed_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoubleComplex* d_S, double* d_W,
                                     hipDoubleComplex* d_U, int* d_info, int batch_size, int max_iter,
                                     double tol, int* d_iters, int* h_fallback, hipStream_t stream) {
-    return run_mixed(reinterpret_cast<const cplx*>(d_H), reinterpret_cast<const cplx*>(d_S), d_W,
-                     reinterpret_cast<cplx*>(d_U), d_info, batch_size, max_iter, tol, d_iters, h_fallback, stream);
+    return run_mixed<false>(reinterpret_cast<const cplx*>(d_H), reinterpret_cast<const cplx*>(d_S), d_W,
+                            reinterpret_cast<cplx*>(d_U), d_info, batch_size, max_iter, tol, d_iters, h_fallback,
+                            stream);
+}
+
+int geneig_mixed_eta_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U, int* d_info,
+                                        int batch_size, int max_iter, double tol, int* d_iters, int* h_fallback,
+                                        hipStream_t stream) {
+    return run_mixed<true>(nullptr, reinterpret_cast<const cplx*>(d_S), d_W, reinterpret_cast<cplx*>(d_U), d_info,
+                           batch_size, max_iter, tol, d_iters, h_fallback, stream);
 }
 
 int geneig_mixed_default_max_iter(void) { return kMaxIter; }

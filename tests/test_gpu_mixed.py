@@ -324,3 +324,49 @@ def test_mixed_graph_capture(geneig, oracle, device):
         assert int(i0.abs().max()) == 0
         assert fb0 == (total if kw else 0)
         del g
+
+
+@pytest.mark.parametrize("total", [1024, 30000])
+def test_mixed_eta_equals_general_precision_form(geneig, oracle, device, total):
+    """The (eta, S) launcher (A = eta not read; K1f and the fallback K1 form it
+    in registers, the refinement scales X by eta instead of a matrix-core pass)
+    against the general launcher on the materialized spin-wave pairs: W, U,
+    info and the step counts bit for bit, at 1,024 pairs (one launch) and
+    30,000 (three concurrent parts); the outputs meet the FP64 bars against
+    the oracle."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, total, seed=31)
+    eta = torch.zeros(54, dtype=torch.complex128, device=device)
+    eta[:27], eta[27:] = 1.0, -1.0
+    assert torch.equal(A[0], torch.diag(eta))    # the generator's A is eta
+    W, U, info = geneig.empty_outputs(54, total, device)
+    it = torch.full((total,), -1, dtype=torch.int32, device=device)
+    fb = geneig.mixed_n54_batched(A, B, W, U, info, iters=it)
+    W2, U2, i2 = geneig.empty_outputs(54, total, device)
+    it2 = torch.full((total,), -1, dtype=torch.int32, device=device)
+    fb2 = geneig.mixed_eta_n54_batched(B, W2, U2, i2, iters=it2)
+    torch.cuda.synchronize()
+    print("eta form, %d pairs: fallback %d / %d, steps %s" % (total, fb2, fb, hist(to_host(it2))), flush=True)
+    assert fb == fb2 and torch.equal(it, it2) and torch.equal(info, i2)
+    assert torch.equal(W, W2) and torch.equal(U, U2)
+    if total == 1024:
+        Ah, Bh = to_host(A), to_host(B)
+        Wo, Xo, io = oracle.zhegv(Ah, Bh)
+        check_pairs(oracle, Ah, Bh, to_host(W2), to_host(U2), Wo, Xo, 54, "eta form 1024")
+
+
+def test_mixed_eta_fallback_matches_fp64_bitwise(geneig, oracle, device):
+    """Every pair handed to the FP64 fallback (tolerance no refinement meets):
+    its K1 factors S and forms eta in registers; the outputs equal the FP64
+    drop-in's on (eta, S) bit for bit."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 40, seed=3)
+    W, U, info = geneig.empty_outputs(54, 40, device)
+    fb = geneig.mixed_eta_n54_batched(B, W, U, info, max_iter=1, tol=1e-300)
+    W2, U2, i2 = geneig.empty_outputs(54, 40, device)
+    geneig.full_n54_batched(A, B, W2, U2, i2)
+    torch.cuda.synchronize()
+    assert fb == 40
+    assert torch.equal(info, i2) and torch.equal(W, W2) and torch.equal(U, U2)
