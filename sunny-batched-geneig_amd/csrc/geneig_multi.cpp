@@ -8,6 +8,7 @@
 // and the graph holds kernel nodes only (no memset nodes).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -88,10 +89,20 @@ int capture(geneig_multi_t m, const int* counts, const void* const* A, const voi
         const int rc = geneig_solver_reserve(m->solvers[k], counts[k], m->branch[k]);
         if (rc) return rc;
     }
+    // Branches in order of decreasing work (n^3 per pair times the count): the
+    // graph's nodes are submitted in capture order, so the bin that takes
+    // longest claims the CUs first instead of trailing the small bins.
+    std::vector<size_t> order(K);
+    for (size_t k = 0; k < K; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const double na = m->sizes[a], nb = m->sizes[b];
+        return na * na * na * counts[a] > nb * nb * nb * counts[b];
+    });
     GENEIG_HIP_TRY(hipStreamBeginCapture(m->cap, hipStreamCaptureModeThreadLocal));
     int rc = GENEIG_E_OK;
     hipError_t e = hipEventRecord(m->fork, m->cap);
-    for (size_t k = 0; k < K && e == hipSuccess && rc == GENEIG_E_OK; ++k) {
+    for (size_t i = 0; i < K && e == hipSuccess && rc == GENEIG_E_OK; ++i) {
+        const size_t k = order[i];
         if (counts[k] == 0) continue;
         e = hipStreamWaitEvent(m->branch[k], m->fork, 0);
         if (e != hipSuccess) break;
