@@ -468,7 +468,10 @@ class GeneigMixedSolver(GeneigSolver):
                                                     int(cache_mode)),
                "GeneigMixedSolver(n=%d, block_dim_x=%d)" % (n, block_dim_x))
 
-    def launch_mixed(self, H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None):
+    def launch_mixed(self, H, S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None,
+                     report_fallback=True):
+        """Returns the number of pairs re-solved in FP64, which waits for the
+        launch; with report_fallback=False nothing waits and None is returned."""
         batch = H.shape[0] if batch is None else batch
         _check_batch(self.matrix_size(), batch, H, S, W, U, info)
         if iters is not None:
@@ -477,9 +480,10 @@ class GeneigMixedSolver(GeneigSolver):
                 raise GeneigError("iters holds %d entries, batch is %d" % (iters.numel(), batch))
         fb = C.c_int(0)
         _check(lib().geneig_mixed_solver_launch(self._h, _ptr(H), _ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
-                                                int(max_iter), float(tol), _ptr(iters), C.byref(fb), _stream(stream)),
+                                                int(max_iter), float(tol), _ptr(iters),
+                                                C.byref(fb) if report_fallback else None, _stream(stream)),
                "GeneigMixedSolver.launch_mixed")
-        return fb.value
+        return fb.value if report_fallback else None
 
 
 class GeneigMulti:

@@ -370,3 +370,47 @@ def test_mixed_eta_fallback_matches_fp64_bitwise(geneig, oracle, device):
     torch.cuda.synchronize()
     assert fb == 40
     assert torch.equal(info, i2) and torch.equal(W, W2) and torch.equal(U, U2)
+
+
+def test_mixed_eta_not_positive_definite(geneig, oracle, device):
+    """An S whose leading minor of order 11 is not positive definite: the
+    (eta, S) launcher reports info = n + 11 (zhegv numbering) and NaN for that
+    pair, as the general launcher does on (eta, S); the other pairs' outputs
+    equal the general launcher's bit for bit."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 64, seed=17)
+    B[5, 10, 10] = -1.0
+    W, U, info = geneig.empty_outputs(54, 64, device)
+    geneig.mixed_n54_batched(A, B, W, U, info)
+    W2, U2, i2 = geneig.empty_outputs(54, 64, device)
+    geneig.mixed_eta_n54_batched(B, W2, U2, i2)
+    torch.cuda.synchronize()
+    i2h = to_host(i2)
+    assert i2h[5] == 54 + 11 and (np.delete(i2h, 5) == 0).all(), i2h
+    assert torch.equal(info, i2)
+    assert torch.isnan(W2[5]).all() and torch.isnan(U2[5].real).all()
+    keep = [p for p in range(64) if p != 5]
+    assert torch.equal(W[keep], W2[keep]) and torch.equal(U[keep], U2[keep])
+
+
+def test_runtime_precision_launch_without_fallback_count(geneig, oracle, device):
+    """The runtime precision launch with no host count (h_fallback NULL: the
+    launch never waits) gives the same outputs, bit for bit, as one that
+    reports the count; with every pair handed back (tol no step meets) the
+    count read afterwards on the same inputs is the batch."""
+    import torch
+
+    A, B = oracle.gen_generic(40, 24, seed=21)
+    dA, dB = to_dev(A, device), to_dev(B, device)
+    s = geneig.GeneigMixedSolver(40, cache_mode=geneig.CACHE_NODISK)
+    W, U, info = geneig.empty_outputs(40, 24, device)
+    assert s.launch_mixed(dA, dB, W, U, info, report_fallback=False) is None
+    W2, U2, i2 = geneig.empty_outputs(40, 24, device)
+    fb = s.launch_mixed(dA, dB, W2, U2, i2)
+    torch.cuda.synchronize()
+    assert fb == 0
+    assert torch.equal(W, W2) and torch.equal(U, U2) and torch.equal(info, i2)
+    W3, U3, i3 = geneig.empty_outputs(40, 24, device)
+    assert s.launch_mixed(dA, dB, W3, U3, i3, max_iter=1, tol=1e-300, report_fallback=False) is None
+    assert s.launch_mixed(dA, dB, W3, U3, i3, max_iter=1, tol=1e-300) == 24
