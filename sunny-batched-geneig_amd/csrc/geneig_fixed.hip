@@ -249,8 +249,8 @@ hipError_t pipeline(const geneig::cplx* a, const geneig::cplx* b, double* W, gen
 template <int MODE>
 hipError_t run(const void* A, const void* B, double* W, void* U, int* info, int batch, hipStream_t stream,
                unsigned char* fixed_ws = nullptr, size_t fixed_bytes = 0) {
-    RETURN_IF(set_attrs<MODE>());
     if (batch <= 0) return hipSuccess;
+    RETURN_IF(set_attrs<MODE>());
     geneig_host::PlainLayout L = fixed_ws ? geneig_host::plain_layout(batch, W54::PAIR_BYTES, 1) : layout_of<MODE>(batch);
     void* ws = nullptr;
     if (fixed_ws) {

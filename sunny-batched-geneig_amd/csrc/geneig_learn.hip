@@ -651,7 +651,7 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
                                       int max_iter, double tol, int* d_sweeps, int* d_iters, int* h_fallback,
                                       hipStream_t stream) {
     if (h_fallback) *h_fallback = 0;
-    if (batch < 0) return hipErrorInvalidValue;
+    if (batch <= 0) return hipSuccess;   // the library's rule: no work, no error
     if (d_features && !d_weights) return hipErrorInvalidValue;
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1c_kernel), kK1LdsF));
@@ -660,7 +660,6 @@ int geneig_learned_n54_batched_launch(const hipDoubleComplex* d_H, const hipDoub
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel), RC::LDS));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel), kK1LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
-    if (batch == 0) return hipSuccess;
     if (max_iter <= 0) max_iter = 4;
     if (!(tol > 0.0)) tol = 1e-7;
     if (max_sweeps < 0) max_sweeps = 0;

@@ -200,7 +200,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
                      double tol, int* iters, int* h_fallback, hipStream_t stream) {
     if (ETA) A = B;
     if (h_fallback) *h_fallback = 0;
-    if (batch < 0) return hipErrorInvalidValue;
+    if (batch <= 0) return hipSuccess;   // the library's rule: no work, no error
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel<ETA>), kK1LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), kK3LdsF));
@@ -209,7 +209,6 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2af_kernel), kK2aLdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2al_kernel), kK2aLdsD));
-    if (batch == 0) return hipSuccess;
     if (maxit <= 0) maxit = kMaxIter;
     if (!(tol > 0.0)) tol = kTol;
 

@@ -208,3 +208,25 @@ def test_rtc_refuses_non_cdna_target(geneig):
     with a message naming the requirement instead of producing wrong code."""
     with pytest.raises(geneig.GeneigError, match="gfx9"):
         geneig.rtc_compile(54, 1, variant=1, arch="gfx1100")
+
+
+@pytest.mark.parametrize("batch", [0, -5])
+def test_launchers_treat_empty_batches_as_no_ops(geneig, batch):
+    """Every batched launcher returns 0 for batch <= 0 before any HIP call
+    (here with null pointers on a host without a device), as the library's
+    contract and the reference's launchers (batch <= 0 returns early) have it;
+    a requested fallback count reads 0."""
+    L = geneig.lib()
+    fb = C.c_int(7)
+    assert L.geneig_full_n54_batched_launch(None, None, None, None, None, batch, None) == 0
+    assert L.geneig_full_n54_batched_inplace_launch(None, None, None, None, batch, None) == 0
+    assert L.geneig_mixed_n54_batched_launch(None, None, None, None, None, batch, 0, C.c_double(0.0), None,
+                                             C.byref(fb), None) == 0 and fb.value == 0
+    fb.value = 7
+    assert L.geneig_mixed_eta_n54_batched_launch(None, None, None, None, batch, 0, C.c_double(0.0), None,
+                                                 C.byref(fb), None) == 0 and fb.value == 0
+    fb.value = 7
+    assert L.geneig_learned_n54_batched_launch(None, None, None, None, None, batch, None, None, 0,
+                                               C.c_float(0.0), 0, C.c_double(0.0), None, None, C.byref(fb),
+                                               None) == 0 and fb.value == 0
+    assert L.geneig_spinwave_n54_launch(0, None, None, None, None, None, None, None, None, batch, None) == 0
