@@ -28,6 +28,7 @@ def solve(geneig, s, A, B, device, inplace=False):
     n = A.shape[-1]
     dA, dB = to_dev(A, device), to_dev(B, device)
     W, U, info = geneig.empty_outputs(n, A.shape[0], device)
+    info.fill_(-1)                 # every pair's info is written by the solve
     if inplace:
         s.launch_inplace(dA, dB, W, info)
         U = dA
@@ -124,6 +125,7 @@ def test_tile_precision_form(geneig, oracle, device, n, kind):
     s = geneig.GeneigMixedSolver(n, cache_mode=geneig.CACHE_NODISK, block_dim_x=TILE)
     assert s.block_dim()[0] == TILE
     W, U, info = geneig.empty_outputs(n, A.shape[0], device)
+    info.fill_(-1)
     fb = s.launch_mixed(A, B, W, U, info)
     torch.cuda.synchronize()
     Wo, Xo, io = oracle.zhegv(Ah, Bh)
