@@ -600,6 +600,24 @@ def bench_spinwave_fused(args, geneig, torch, dev):
     print(json.dumps(line), flush=True)
 
 
+def paired_exact_count(torch, W, U, chunk=16384):
+    """Pairs whose outputs carry the spin-wave pairing bit for bit: W[n-1-i] =
+    -W[i] and X(:, i) = Sigma conj X(:, n-1-i), Sigma swapping the halves of
+    a column (storage is column-major: U[p, col, row])."""
+    h = U.shape[-1] // 2
+    cnt = 0
+    for off in range(0, W.shape[0], chunk):
+        w, u = W[off:off + chunk], U[off:off + chunk]
+        part = torch.cat([u[:, :, h:], u[:, :, :h]], dim=2).conj().flip(1)
+        cnt += int(((w + w.flip(1) == 0).all(dim=1) & (u == part).flatten(1).all(dim=1)).sum().item())
+    return cnt
+
+
+def max_rel_dw(torch, W1, W2):
+    """max over pairs of max_i |W1 - W2| / max(max_i |W1|, 1)."""
+    return float(((W1 - W2).abs().amax(dim=1) / W1.abs().amax(dim=1).clamp(min=1.0)).max().item())
+
+
 def bench_spinwave_eta(args, geneig, torch, dev):
     """The precision form for the pencil (eta, S) (geneig_mixed_eta_n54_batched_launch:
     A = eta not read, the refinement's A X a row scaling) against the general
@@ -627,9 +645,17 @@ def bench_spinwave_eta(args, geneig, torch, dev):
     te = timed(lambda: geneig.mixed_eta_n54_batched(B, W2, U2, i2, iters=it2, stream=stream, report_fallback=False))
     same = bool(torch.equal(W, W2) and torch.equal(U, U2) and torch.equal(info, i2) and torch.equal(it, it2))
     dev_e = device_check(torch, A, B, W2, U2, i2)
+    # the paired launcher on the same pairs (S of H(q)'s block form): refines
+    # the 27 positive eigenpairs, writes their partners
+    tq = timed(lambda: geneig.mixed_eta_paired_n54_batched(B, W, U, info, stream=stream, report_fallback=False))
+    fb_q = geneig.mixed_eta_paired_n54_batched(B, W, U, info, stream=stream)
+    torch.cuda.synchronize()
+    dev_q = device_check(torch, A, B, W, U, info)
+    paired_q = paired_exact_count(torch, W, U)
+    dw_q = max_rel_dw(torch, W2, W)
     tp = timed(lambda: geneig.full_n54_batched(A, B, W, U, info, stream=stream))
-    log("eta form %.2f ms, general precision form %.2f ms, plain FP64 %.2f ms, outputs equal: %s"
-        % (te * 1e3, tg * 1e3, tp * 1e3, same))
+    log("eta form %.2f ms, paired %.2f ms, general precision form %.2f ms, plain FP64 %.2f ms, outputs equal: %s"
+        % (te * 1e3, tq * 1e3, tg * 1e3, tp * 1e3, same))
     v, c = np.unique(it2.cpu().numpy(), return_counts=True)
     line = {
         "metric": "n=54 spin-wave solves/s, precision form for the pencil (eta, S) (A = eta not read)",
@@ -642,6 +668,10 @@ def bench_spinwave_eta(args, geneig, torch, dev):
         "outputs_equal_general_precision_form": same, "device_full_batch": dev_e,
         "refine_steps_histogram": {str(int(a)): int(b) for a, b in zip(v, c)},
         "general_precision_form": {"value": batch / tg, "ms_per_step": tg * 1e3},
+        "paired_form": {"value": batch / tq, "ms_per_step": tq * 1e3,
+                        "entry": "geneig_mixed_eta_paired_n54_batched_launch",
+                        "fallback_pairs": fb_q, "exactly_paired": paired_q, "device_full_batch": dev_q,
+                        "max_rel_eig_diff_vs_eta_form": dw_q},
         "plain_fp64": {"value": batch / tp, "ms_per_step": tp * 1e3},
         "speedup_vs_general": tg / te, "speedup_vs_plain": tp / te,
     }
@@ -835,9 +865,20 @@ def main():
                                                                   report_fallback=False))
     torch.cuda.synchronize()
     eta_same = bool(torch.equal(W, We) and torch.equal(U, Ue) and torch.equal(info, ie))
-    del We, Ue, ie
     log("eta form: %.3f ms/step, launch avg %.3f ms, outputs equal the chosen form's: %s"
         % (wall_e * 1e3 / args.steps, launch_e * 1e3, eta_same))
+    # the paired launcher (S of H(q)'s block form: the refinement works on the
+    # 27 positive eigenpairs and writes their partners) on the same pairs
+    wall_q, launch_q = timed(lambda: geneig.mixed_eta_paired_n54_batched(B, We, Ue, ie, stream=stream,
+                                                                         report_fallback=False))
+    fb_q = geneig.mixed_eta_paired_n54_batched(B, We, Ue, ie, stream=stream)
+    torch.cuda.synchronize()
+    dev_q = device_check(torch, A, B, We, Ue, ie)
+    paired_q = paired_exact_count(torch, We, Ue)
+    dw_q = max_rel_dw(torch, W, We)
+    del We, Ue, ie
+    log("paired form: %.3f ms/step, launch avg %.3f ms, fallback %d, exactly paired %d, |dW| vs chosen %.2e, "
+        "device check %s" % (wall_q * 1e3 / args.steps, launch_q * 1e3, fb_q, paired_q, dw_q, dev_q))
 
     value = batch / (wall_c / args.steps)
     plain_value = batch / (wall_p / args.steps)
@@ -938,6 +979,13 @@ def main():
                      "entry": "geneig_mixed_eta_n54_batched_launch (A = eta not read)",
                      "outputs_equal_chosen_form": eta_same,
                      "speedup_vs_plain": (batch / (wall_e / args.steps)) / plain_value},
+        "paired_form": {"value": batch / (wall_q / args.steps), "unit": "solves/s",
+                        "ms_per_step": wall_q * 1e3 / args.steps, "avg_launch_ms": launch_q * 1e3,
+                        "entry": "geneig_mixed_eta_paired_n54_batched_launch (A = eta, S = [[P, Q], [conj Q, "
+                                 "conj P]]: refinement of the 27 positive eigenpairs, partners written)",
+                        "fallback_pairs": fb_q, "exactly_paired": paired_q, "device_full_batch": dev_q,
+                        "max_rel_eig_diff_vs_chosen": dw_q,
+                        "speedup_vs_plain": (batch / (wall_q / args.steps)) / plain_value},
         "roofline": roofline,
         "cpu_baseline": (dict(lapack, kind="reference",
                               routine="LAPACK zhegv (itype 1, 'V', 'L'), the solver of the reference's CPU "

@@ -54,6 +54,22 @@ int geneig_mixed_eta_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W
                                         int* d_info, int batch_size, int max_iter, double tol, int* d_iters,
                                         int* h_fallback, hipStream_t stream);
 
+/* The same pencil when S has the block form of a spin-wave Hamiltonian,
+ * S = [[P, Q], [conj(Q), conj(P)]] (27 x 27 blocks, as Sunny's H(q) has):
+ * its eigenpairs then come in pairs (lambda, x), (-lambda, Sigma conj(x)),
+ * Sigma swapping the two halves of x, and the FP64 refinement works on the 27
+ * positive ones only (half the matrix-core products per pair, twice the
+ * pairs per CU) and writes their partners.  Every pair the refinement
+ * finishes has the pairing exactly: W[53 - i] = -W[i], X(:, i) =
+ * Sigma conj(X(:, 53 - i)); pairs it hands back are solved by the FP64 path
+ * as there, and carry the pairing to rounding and phase.  The outputs meet
+ * the same bars as geneig_mixed_eta_n54_batched_launch's, from which they
+ * differ by rounding.  The block form is not checked: an S without it gets
+ * wrong results.  No reference counterpart. */
+int geneig_mixed_eta_paired_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U,
+                                               int* d_info, int batch_size, int max_iter, double tol,
+                                               int* d_iters, int* h_fallback, hipStream_t stream);
+
 /* Precision form for any n (1..100): a runtime solver (geneig_solver.h) whose
  * hipRTC module holds the precision-form kernels specialized for n (cached
  * like the FP64 module, variant 1 in the cache key).  geneig_solver_launch on

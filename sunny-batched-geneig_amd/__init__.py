@@ -59,6 +59,7 @@ def lib():
         "geneig_finish_n54_launch": (I, [P, P, P, P, P, I, P]),
         "geneig_mixed_n54_batched_launch": (I, [P, P, P, P, P, I, I, D, P, P, P]),
         "geneig_mixed_eta_n54_batched_launch": (I, [P, P, P, P, I, I, D, P, P, P]),
+        "geneig_mixed_eta_paired_n54_batched_launch": (I, [P, P, P, P, I, I, D, P, P, P]),
         "geneig_mixed_default_max_iter": (I, []),
         "geneig_mixed_default_tol": (D, []),
         "geneig_mixed_solver_create": (I, [P, I, I, I]),
@@ -279,6 +280,28 @@ def mixed_eta_n54_batched(S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=
                                                          _ptr(iters) if iters is not None else None,
                                                          C.byref(fb) if report_fallback else None, _stream(stream)),
                "geneig_mixed_eta_n54_batched_launch")
+    return fb.value if report_fallback else None
+
+
+def mixed_eta_paired_n54_batched(S, W, U, info, batch=None, max_iter=0, tol=0.0, iters=None, stream=None,
+                                 report_fallback=True):
+    """The precision form for the spin-wave pencil (eta, S) with S = [[P, Q],
+    [conj Q, conj P]] (geneig_mixed_eta_paired_n54_batched_launch): the
+    refinement works on the 27 positive eigenpairs and writes their partners
+    (-lambda, Sigma conj(x)); arguments and outputs as ``mixed_eta_n54_batched``."""
+    batch = S.shape[0] if batch is None else batch
+    _check_batch(54, batch, S, S, W, U, info)
+    if iters is not None:
+        _require(iters, _torch().int32, "iters")
+        if iters.numel() < batch:
+            raise GeneigError("iters holds %d entries, batch is %d" % (iters.numel(), batch))
+    fb = C.c_int(0)
+    _check_hip(lib().geneig_mixed_eta_paired_n54_batched_launch(_ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
+                                                                int(max_iter), float(tol),
+                                                                _ptr(iters) if iters is not None else None,
+                                                                C.byref(fb) if report_fallback else None,
+                                                                _stream(stream)),
+               "geneig_mixed_eta_paired_n54_batched_launch")
     return fb.value if report_fallback else None
 
 

@@ -119,6 +119,22 @@ __global__ void __launch_bounds__(64 * RC::NT, GENEIG_REFINE_WAVES_PER_EU) refin
     geneig::refine_batched<kN, ETA>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem, first);
 }
 
+// The refinement of the spin-wave pencil's positive half (PAIRED): two waves
+// per pair, n/2 columns of X in LDS, so six pairs per CU.
+using RCP = geneig::RCfg<kN, true>;
+#ifndef GENEIG_REFINE_PAIRED_WAVES_PER_EU
+#define GENEIG_REFINE_PAIRED_WAVES_PER_EU (geneig::refine_waves_per_eu<kN, true>())
+#endif
+__global__ void __launch_bounds__(64 * RCP::NW, GENEIG_REFINE_PAIRED_WAVES_PER_EU)
+    refine_paired_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B, double* __restrict__ W,
+                         cplx* __restrict__ U, const int* __restrict__ info, int* __restrict__ fb_list,
+                         int* __restrict__ fb_count, int* __restrict__ iters, int batch, int maxit, double tol,
+                         int first) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    geneig::refine_batched<kN, true, true>(A, B, W, U, info, fb_list, fb_count, iters, batch, maxit, tol, smem,
+                                           first);
+}
+
 // The FP64 pipeline over the fallback list: round `off` covers entries off ..
 // off + chunk - 1 of the list, whose length the kernels read on the device
 // (dispatch_slots), so the host never waits for it.
@@ -194,17 +210,22 @@ hipError_t set_lds_once(const void* kernel, unsigned bytes) {
     } while (0)
 
 // ETA: A = eta, not read (A may be any pointer; the kernels that would read it
-// get B in its place).
-template <bool ETA>
+// get B in its place).  PAIRED (with ETA): the refinement of the positive
+// half of the spin-wave pencil's eigenpairs (refine_pair).
+template <bool ETA, bool PAIRED = false>
 hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info, int batch, int maxit,
                      double tol, int* iters, int* h_fallback, hipStream_t stream) {
+    static_assert(!PAIRED || ETA, "the paired refinement is for the pencil (eta, S)");
     if (ETA) A = B;
     if (h_fallback) *h_fallback = 0;
     if (batch <= 0) return hipSuccess;   // the library's rule: no work, no error
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel<ETA>), kK1LdsF));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), kK3LdsF));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel<ETA>), RC::LDS));
+    if (PAIRED)
+        RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_paired_kernel), RCP::LDS));
+    else
+        RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_kernel<ETA>), RC::LDS));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1l_kernel<ETA>), kK1LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3l_kernel), kK3LdsD));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2af_kernel), kK2aLdsF));
@@ -245,6 +266,16 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
                            W + (size_t)kN * off, U + mat * off, info + off, w, P);
         return hipGetLastError();
     };
+    // the refinement of pairs [first, end) on stream st, one workgroup per pair
+    auto refine = [&](unsigned grid, hipStream_t st, int end, int first) -> hipError_t {
+        if constexpr (PAIRED)
+            hipLaunchKernelGGL(refine_paired_kernel, dim3(grid), dim3(64 * RCP::NW), RCP::LDS, st, A, B, W, U, info,
+                               fb_list, fb_count, iters, end, maxit, tol, first);
+        else
+            hipLaunchKernelGGL(refine_kernel<ETA>, dim3(grid), dim3(64 * RC::NT), RC::LDS, st, A, B, W, U, info,
+                               fb_list, fb_count, iters, end, maxit, tol, first);
+        return hipGetLastError();
+    };
     if (L.parts > 1) {
         hipStream_t subs[geneig_host::kMaxParts];
         RETURN_IF(geneig_host::fork_streams(stream, L.parts, subs));
@@ -256,11 +287,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
                                B + mat * off, W + (size_t)kN * off, U + mat * off, info + off, P);
             e = hipGetLastError();
             if (e == hipSuccess) e = bulk(off, P, solver + L.region_f * i, subs[i]);
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(refine_kernel<ETA>, dim3((unsigned)P), dim3(64 * RC::NT), RC::LDS, subs[i], A, B, W, U,
-                                   info, fb_list, fb_count, iters, off + P, maxit, tol, off);
-                e = hipGetLastError();
-            }
+            if (e == hipSuccess) e = refine((unsigned)P, subs[i], off + P, off);
         }
         const hipError_t j = geneig_host::join_streams(stream, L.parts, subs);
         RETURN_IF(e);
@@ -271,9 +298,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         RETURN_IF(hipGetLastError());
         for (int off = 0; off < batch; off += L.chunk_f)
             RETURN_IF(bulk(off, std::min(L.chunk_f, batch - off), solver, stream));
-        hipLaunchKernelGGL(refine_kernel<ETA>, dim3((unsigned)batch), dim3(64 * RC::NT), RC::LDS, stream, A, B, W, U, info,
-                           fb_list, fb_count, iters, batch, maxit, tol, 0);
-        RETURN_IF(hipGetLastError());
+        RETURN_IF(refine((unsigned)batch, stream, batch, 0));
     }
 
     // The FP64 re-solve of the listed pairs, in rounds of chunk_d list
@@ -339,6 +364,13 @@ int geneig_mixed_eta_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W
                                         hipStream_t stream) {
     return run_mixed<true>(nullptr, reinterpret_cast<const cplx*>(d_S), d_W, reinterpret_cast<cplx*>(d_U), d_info,
                            batch_size, max_iter, tol, d_iters, h_fallback, stream);
+}
+
+int geneig_mixed_eta_paired_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U,
+                                               int* d_info, int batch_size, int max_iter, double tol, int* d_iters,
+                                               int* h_fallback, hipStream_t stream) {
+    return run_mixed<true, true>(nullptr, reinterpret_cast<const cplx*>(d_S), d_W, reinterpret_cast<cplx*>(d_U),
+                                 d_info, batch_size, max_iter, tol, d_iters, h_fallback, stream);
 }
 
 int geneig_mixed_default_max_iter(void) { return kMaxIter; }

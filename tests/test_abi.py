@@ -226,6 +226,9 @@ def test_launchers_treat_empty_batches_as_no_ops(geneig, batch):
     assert L.geneig_mixed_eta_n54_batched_launch(None, None, None, None, batch, 0, C.c_double(0.0), None,
                                                  C.byref(fb), None) == 0 and fb.value == 0
     fb.value = 7
+    assert L.geneig_mixed_eta_paired_n54_batched_launch(None, None, None, None, batch, 0, C.c_double(0.0), None,
+                                                        C.byref(fb), None) == 0 and fb.value == 0
+    fb.value = 7
     assert L.geneig_learned_n54_batched_launch(None, None, None, None, None, batch, None, None, 0,
                                                C.c_float(0.0), 0, C.c_double(0.0), None, None, C.byref(fb),
                                                None) == 0 and fb.value == 0

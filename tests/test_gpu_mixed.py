@@ -414,3 +414,124 @@ def test_runtime_precision_launch_without_fallback_count(geneig, oracle, device)
     W3, U3, i3 = geneig.empty_outputs(40, 24, device)
     assert s.launch_mixed(dA, dB, W3, U3, i3, max_iter=1, tol=1e-300, report_fallback=False) is None
     assert s.launch_mixed(dA, dB, W3, U3, i3, max_iter=1, tol=1e-300) == 24
+
+
+def paired_exact(W, U):
+    """Per pair: whether W[53 - i] = -W[i] and X(:, i) = Sigma conj X(:, 53 - i)
+    hold bit for bit (U column-major: U[p, col, row]; Sigma swaps rows
+    0..26 and 27..53)."""
+    import torch
+
+    w_ok = (W + W.flip(1) == 0).all(dim=1)
+    part = torch.cat([U[:, :, 27:], U[:, :, :27]], dim=2).conj().flip(1)
+    return w_ok & (U == part).flatten(1).all(dim=1)
+
+
+def spin_resid_orth(A, B, W, U):
+    """Largest scale-invariant residual and B-orthogonality defect (units of
+    n eps) over the pairs, on the device, in 4096-pair chunks; storage is
+    column-major, so with Xt = X^T: (A X)^T = Xt At, (X^H B X)^T = (Xt Bt)
+    conj(Xt)^T."""
+    import torch
+
+    eps = float(np.finfo(np.float64).eps)
+    res = orth = 0.0
+    eye = torch.eye(54, dtype=U.dtype, device=U.device)[None]
+    for s in range(0, A.shape[0], 4096):
+        At, Bt, Xt, w = A[s:s + 4096], B[s:s + 4096], U[s:s + 4096], W[s:s + 4096]
+        XB = torch.matmul(Xt, Bt)
+        R = torch.matmul(Xt, At) - w[:, :, None] * XB
+        xn = torch.linalg.vector_norm(Xt, dim=2)
+        fa, fb = torch.linalg.matrix_norm(At), torch.linalg.matrix_norm(Bt)
+        res = max(res, float((torch.linalg.vector_norm(R, dim=2) /
+                              ((fa[:, None] + w.abs() * fb[:, None]) * xn)).max()))
+        G = torch.matmul(XB, Xt.conj().transpose(-1, -2).contiguous()) - eye
+        orth = max(orth, float((G.abs() / (fb[:, None, None] * xn[:, :, None] * xn[:, None, :])).max()))
+    return res / (54 * eps), orth / (54 * eps)
+
+
+def test_mixed_eta_paired_vs_oracle(geneig, oracle, device):
+    """The paired launcher on spin-wave pairs (S = [[P, Q], [conj Q, conj P]]
+    as the generator builds H(q)): FP64 bars against the oracle, and every
+    pair the refinement finished carries the pairing bit for bit (the FP64
+    fallback's pairs carry it to rounding and phase)."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 1024, seed=1, first_q=5000)
+    W, U, info = geneig.empty_outputs(54, 1024, device)
+    info.fill_(-1)
+    it = torch.full((1024,), -1, dtype=torch.int32, device=device)
+    fb = geneig.mixed_eta_paired_n54_batched(B, W, U, info, iters=it)
+    torch.cuda.synchronize()
+    ok = paired_exact(W, U)
+    print("paired form 1024: fallback %d, steps %s, exactly paired %d" % (fb, hist(to_host(it)), int(ok.sum())),
+          flush=True)
+    assert int(info.abs().max()) == 0
+    assert fb <= 1024 // 20 and int(ok.sum()) >= 1024 - fb
+    Ah, Bh = to_host(A), to_host(B)
+    Wo, Xo, io = oracle.zhegv(Ah, Bh)
+    check_pairs(oracle, Ah, Bh, to_host(W), to_host(U), Wo, Xo, 54, "paired form 1024")
+
+
+@pytest.mark.parametrize("total", [30000])
+def test_mixed_eta_paired_close_to_eta_form(geneig, oracle, device, total):
+    """30,000 spin-wave pairs (three concurrent parts): the paired launcher's
+    eigenvalues agree with the eta form's to 1e-12 relative to the largest
+    |lambda|, info agrees, residual and B-orthogonality of every pair stay
+    <= 50 n eps, and all but the fallback pairs are paired bit for bit."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, total, seed=31)
+    W, U, info = geneig.empty_outputs(54, total, device)
+    fb = geneig.mixed_eta_n54_batched(B, W, U, info)
+    W2, U2, i2 = geneig.empty_outputs(54, total, device)
+    i2.fill_(-1)
+    fb2 = geneig.mixed_eta_paired_n54_batched(B, W2, U2, i2)
+    torch.cuda.synchronize()
+    ok = paired_exact(W2, U2)
+    dw = float(((W - W2).abs().amax(dim=1) / W.abs().amax(dim=1).clamp(min=1.0)).max())
+    res, orth = spin_resid_orth(A, B, W2, U2)
+    print("paired form %d: fallback %d (eta form %d), exactly paired %d, |dW| %.2e, resid %.3f, orth %.3f n eps"
+          % (total, fb2, fb, int(ok.sum()), dw, res, orth), flush=True)
+    assert torch.equal(info, i2) and int(info.abs().max()) == 0
+    assert fb2 <= total // 20 and int(ok.sum()) >= total - fb2
+    assert dw <= 1e-12 and res <= 50 and orth <= 50
+
+
+def test_mixed_eta_paired_fallback_matches_fp64_bitwise(geneig, oracle, device):
+    """Every pair handed to the FP64 fallback (tolerance no refinement step
+    meets): the outputs equal the FP64 drop-in's on (eta, S) bit for bit."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 40, seed=3)
+    W, U, info = geneig.empty_outputs(54, 40, device)
+    fb = geneig.mixed_eta_paired_n54_batched(B, W, U, info, max_iter=1, tol=1e-300)
+    W2, U2, i2 = geneig.empty_outputs(54, 40, device)
+    geneig.full_n54_batched(A, B, W2, U2, i2)
+    torch.cuda.synchronize()
+    assert fb == 40
+    assert torch.equal(info, i2) and torch.equal(W, W2) and torch.equal(U, U2)
+
+
+def test_mixed_eta_paired_not_positive_definite(geneig, oracle, device):
+    """An S whose leading minor of order 11 is not positive definite: info =
+    n + 11 and NaN outputs for that pair, as the eta form reports; the other
+    pairs are solved and paired."""
+    import torch
+
+    A, B = geneig.gen_spinwave(27, 64, seed=17)
+    B[5, 10, 10] = -1.0
+    W, U, info = geneig.empty_outputs(54, 64, device)
+    geneig.mixed_eta_n54_batched(B, W, U, info)
+    W2, U2, i2 = geneig.empty_outputs(54, 64, device)
+    fb = geneig.mixed_eta_paired_n54_batched(B, W2, U2, i2)
+    torch.cuda.synchronize()
+    i2h = to_host(i2)
+    assert i2h[5] == 54 + 11 and (np.delete(i2h, 5) == 0).all(), i2h
+    assert torch.equal(info, i2)
+    assert torch.isnan(W2[5]).all() and torch.isnan(U2[5].real).all()
+    keep = [p for p in range(64) if p != 5]
+    ok = paired_exact(W2[keep], U2[keep])
+    assert int(ok.sum()) >= 63 - fb
+    dw = float(((W[keep] - W2[keep]).abs().amax(dim=1) / W[keep].abs().amax(dim=1).clamp(min=1.0)).max())
+    assert dw <= 1e-12
