@@ -109,6 +109,16 @@ __global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3f_kernel(const 
     geneig::k3_batched<kN, 1, geneig::MODE_MIXED, float>(B, W, U, info, ws, batch, batch, smem, vecs);
 }
 
+// K3f of the paired form: only the positive half's columns (27 .. 53), the
+// ones its refinement reads, are formed and written.
+__global__ void __launch_bounds__(64, GENEIG_K3F_WAVES_PER_EU) k3f_half_kernel(const cplx* __restrict__ B,
+                                                  double* __restrict__ W, cplx* __restrict__ U,
+                                                  const int* __restrict__ info, unsigned char* ws, int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ cplxf vecs[CF::VEC];
+    geneig::k3_batched<kN, 1, geneig::MODE_MIXED, float, false, kN / 2>(B, W, U, info, ws, batch, batch, smem, vecs);
+}
+
 template <bool ETA>
 __global__ void __launch_bounds__(64 * RC::NT, GENEIG_REFINE_WAVES_PER_EU) refine_kernel(const cplx* __restrict__ A, const cplx* __restrict__ B,
                                                               double* __restrict__ W, cplx* __restrict__ U,
@@ -221,7 +231,7 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
     if (batch <= 0) return hipSuccess;   // the library's rule: no work, no error
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(chol64_kernel), geneig::chol64_lds_bytes<kN>()));
     RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1f_kernel<ETA>), kK1LdsF));
-    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3f_kernel), kK3LdsF));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(PAIRED ? k3f_half_kernel : k3f_kernel), kK3LdsF));
     if (PAIRED)
         RETURN_IF(set_lds_once(reinterpret_cast<const void*>(refine_paired_kernel), RCP::LDS));
     else
@@ -262,8 +272,8 @@ hipError_t run_mixed(const cplx* A, const cplx* B, double* W, cplx* U, int* info
         hipLaunchKernelGGL(k2xf_kernel, dim3((unsigned)std::min(P, geneig::kListGridK2x)), dim3(64 * CF::R), 0, st,
                            info + off, w, P, (const int*)ovf);
         RETURN_IF(hipGetLastError());
-        hipLaunchKernelGGL(k3f_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, st, B + mat * off,
-                           W + (size_t)kN * off, U + mat * off, info + off, w, P);
+        hipLaunchKernelGGL(PAIRED ? k3f_half_kernel : k3f_kernel, dim3((unsigned)P), dim3(64), kK3LdsF, st,
+                           B + mat * off, W + (size_t)kN * off, U + mat * off, info + off, w, P);
         return hipGetLastError();
     };
     // the refinement of pairs [first, end) on stream st, one workgroup per pair
