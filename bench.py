@@ -653,9 +653,18 @@ def bench_spinwave_eta(args, geneig, torch, dev):
     dev_q = device_check(torch, A, B, W, U, info)
     paired_q = paired_exact_count(torch, W, U)
     dw_q = max_rel_dw(torch, W2, W)
+    # the real-arithmetic FP64 solve of the same block-form pencil
+    tr = timed(lambda: geneig.eta_paired_n54_batched(B, W, U, info, stream=stream))
+    torch.cuda.synchronize()
+    dev_r = device_check(torch, A, B, W, U, info)
+    paired_r = paired_exact_count(torch, W, U)
+    dw_r = max_rel_dw(torch, W2, W)
+    Wr = W.clone()
     tp = timed(lambda: geneig.full_n54_batched(A, B, W, U, info, stream=stream))
-    log("eta form %.2f ms, paired %.2f ms, general precision form %.2f ms, plain FP64 %.2f ms, outputs equal: %s"
-        % (te * 1e3, tq * 1e3, tg * 1e3, tp * 1e3, same))
+    torch.cuda.synchronize()
+    dw_rp = max_rel_dw(torch, W, Wr)
+    log("eta form %.2f ms, paired %.2f ms, real FP64 %.2f ms, general precision form %.2f ms, plain FP64 %.2f ms, "
+        "outputs equal: %s" % (te * 1e3, tq * 1e3, tr * 1e3, tg * 1e3, tp * 1e3, same))
     v, c = np.unique(it2.cpu().numpy(), return_counts=True)
     line = {
         "metric": "n=54 spin-wave solves/s, precision form for the pencil (eta, S) (A = eta not read)",
@@ -672,6 +681,11 @@ def bench_spinwave_eta(args, geneig, torch, dev):
                         "entry": "geneig_mixed_eta_paired_n54_batched_launch",
                         "fallback_pairs": fb_q, "exactly_paired": paired_q, "device_full_batch": dev_q,
                         "max_rel_eig_diff_vs_eta_form": dw_q},
+        "real_form": {"value": batch / tr, "ms_per_step": tr * 1e3,
+                      "entry": "geneig_eta_paired_n54_batched_launch (real FP64 arithmetic via T = [[I, iI], [I, -iI]]"
+                               " / sqrt 2)",
+                      "exactly_paired": paired_r, "device_full_batch": dev_r,
+                      "max_rel_eig_diff_vs_eta_form": dw_r, "max_rel_eig_diff_vs_plain": dw_rp},
         "plain_fp64": {"value": batch / tp, "ms_per_step": tp * 1e3},
         "speedup_vs_general": tg / te, "speedup_vs_plain": tp / te,
     }
@@ -876,6 +890,14 @@ def main():
     dev_q = device_check(torch, A, B, We, Ue, ie)
     paired_q = paired_exact_count(torch, We, Ue)
     dw_q = max_rel_dw(torch, W, We)
+    # the real-arithmetic FP64 solve of the same block-form pencil
+    wall_r, launch_r = timed(lambda: geneig.eta_paired_n54_batched(B, We, Ue, ie, stream=stream))
+    torch.cuda.synchronize()
+    dev_r = device_check(torch, A, B, We, Ue, ie)
+    paired_r = paired_exact_count(torch, We, Ue)
+    dw_r = max_rel_dw(torch, Wp, We)
+    log("real form: %.3f ms/step, launch avg %.3f ms, exactly paired %d, |dW| vs plain %.2e, device check %s"
+        % (wall_r * 1e3 / args.steps, launch_r * 1e3, paired_r, dw_r, dev_r))
     del We, Ue, ie
     log("paired form: %.3f ms/step, launch avg %.3f ms, fallback %d, exactly paired %d, |dW| vs chosen %.2e, "
         "device check %s" % (wall_q * 1e3 / args.steps, launch_q * 1e3, fb_q, paired_q, dw_q, dev_q))
@@ -986,6 +1008,13 @@ def main():
                         "fallback_pairs": fb_q, "exactly_paired": paired_q, "device_full_batch": dev_q,
                         "max_rel_eig_diff_vs_chosen": dw_q,
                         "speedup_vs_plain": (batch / (wall_q / args.steps)) / plain_value},
+        "real_form": {"value": batch / (wall_r / args.steps), "unit": "solves/s",
+                      "ms_per_step": wall_r * 1e3 / args.steps, "avg_launch_ms": launch_r * 1e3,
+                      "entry": "geneig_eta_paired_n54_batched_launch (A = eta, S = [[P, Q], [conj Q, conj P]] mapped "
+                               "by T = [[I, iI], [I, -iI]] / sqrt 2 to a real pencil, solved in FP64)",
+                      "dtype": "f64 (real)", "exactly_paired": paired_r, "device_full_batch": dev_r,
+                      "max_rel_eig_diff_vs_plain": dw_r,
+                      "speedup_vs_plain": (batch / (wall_r / args.steps)) / plain_value},
         "roofline": roofline,
         "cpu_baseline": (dict(lapack, kind="reference",
                               routine="LAPACK zhegv (itype 1, 'V', 'L'), the solver of the reference's CPU "

@@ -111,6 +111,24 @@ int geneig_full_n54_batched_inplace_launch(hipDoubleComplex* d_H, hipDoubleCompl
 int geneig_spinwave_n54_launch(int nbonds, const int* d_shift, const int* d_r, const double* d_J,
                                const double* d_q, const double* d_diag, double* d_W, hipDoubleComplex* d_U,
                                int* d_info, int batch_size, hipStream_t stream);
+/* The spin-wave pencil (eta, S), eta = diag(I_27, -I_27) (not passed), for S
+ * of the block form of a spin-wave Hamiltonian, S = [[P, Q], [conj Q, conj P]]
+ * (27 x 27 blocks, as H(q) is), solved in real FP64 arithmetic: with
+ * T = [[I, iI], [I, -iI]] / sqrt 2, T^H S T is real symmetric and
+ * T^H eta T = i [[0, I], [-I, 0]], so the Cholesky factor, the reduction, the
+ * Householder tridiagonalization and the back-transform are real (about a
+ * quarter of the complex pipeline's multiplications) and the FP64 QL stages
+ * run on a real tridiagonal with zero diagonal.  The 27 positive eigenpairs
+ * are taken from it and their partners written exactly: W[53 - i] = -W[i],
+ * X(:, i) = Sigma conj X(:, 53 - i), Sigma swapping the halves.  Outputs W, U
+ * as geneig_full_n54_batched_launch's on (eta, S) to FP64 rounding; only the
+ * lower triangle of S is read.  info = 54 + k when the leading minor of
+ * order k of T^H S T (not of S) is not positive definite, with NaN outputs
+ * for that pair.  The block form is not checked: an S without it gets wrong
+ * results.  No reference counterpart (the reference solves the complex
+ * pencil as given). */
+int geneig_eta_paired_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U,
+                                         int* d_info, int batch_size, hipStream_t stream);
 /* The H(q) + diag I the launch above builds, written to d_H (batch x 54 x 54,
  * column-major): for checking the construction, or for callers that need the
  * matrices themselves. */

@@ -60,6 +60,7 @@ def lib():
         "geneig_mixed_n54_batched_launch": (I, [P, P, P, P, P, I, I, D, P, P, P]),
         "geneig_mixed_eta_n54_batched_launch": (I, [P, P, P, P, I, I, D, P, P, P]),
         "geneig_mixed_eta_paired_n54_batched_launch": (I, [P, P, P, P, I, I, D, P, P, P]),
+        "geneig_eta_paired_n54_batched_launch": (I, [P, P, P, P, I, P]),
         "geneig_mixed_default_max_iter": (I, []),
         "geneig_mixed_default_tol": (D, []),
         "geneig_mixed_solver_create": (I, [P, I, I, I]),
@@ -303,6 +304,18 @@ def mixed_eta_paired_n54_batched(S, W, U, info, batch=None, max_iter=0, tol=0.0,
                                                                 _stream(stream)),
                "geneig_mixed_eta_paired_n54_batched_launch")
     return fb.value if report_fallback else None
+
+
+def eta_paired_n54_batched(S, W, U, info, batch=None, stream=None):
+    """The spin-wave pencil (eta, S) with S = [[P, Q], [conj Q, conj P]] in
+    real FP64 arithmetic (geneig_eta_paired_n54_batched_launch): outputs as
+    ``full_n54_batched(eta, S, ...)`` to rounding, with the pairing
+    W[53 - i] = -W[i], X(:, i) = Sigma conj X(:, 53 - i) exact."""
+    batch = S.shape[0] if batch is None else batch
+    _check_batch(54, batch, S, S, W, U, info)
+    _check_hip(lib().geneig_eta_paired_n54_batched_launch(_ptr(S), _ptr(W), _ptr(U), _ptr(info), int(batch),
+                                                          _stream(stream)),
+               "geneig_eta_paired_n54_batched_launch")
 
 
 def full_n54_batched_noalloc(H, S, W, U, info, batch=None, stream=None):

@@ -13,6 +13,7 @@
 #include <set>
 
 #include "geneig_device.inc"
+#include "geneig_real.inc"
 #include "geneig_ws.hpp"
 #include "../../include/geneig.h"
 
@@ -153,6 +154,44 @@ __global__ void __launch_bounds__(64 * kBPB, GENEIG_K3_WAVES_PER_EU) k3_kernel(c
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     __shared__ geneig::cplx vecs[kBPB * C54::VEC];
     geneig::k3_batched<kN, kBPB, MODE>(B, W, U, info, ws, batch, batch, smem, vecs);
+}
+
+// The spin-wave pencil in real arithmetic (geneig_real.inc): K1r and K3r
+// around the FP64 QL stages.
+#ifndef GENEIG_K1R_WAVES_PER_EU
+#define GENEIG_K1R_WAVES_PER_EU 2
+#endif
+#ifndef GENEIG_K3R_WAVES_PER_EU
+#define GENEIG_K3R_WAVES_PER_EU 2
+#endif
+using RC54 = geneig::RealCfg<kN>;
+
+__global__ void __launch_bounds__(64, GENEIG_K1R_WAVES_PER_EU) k1r_kernel(const geneig::cplx* __restrict__ S,
+                                                                          double* __restrict__ W,
+                                                                          geneig::cplx* __restrict__ U,
+                                                                          int* __restrict__ info, unsigned char* ws,
+                                                                          int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ double vecs[2 * RC54::VEC];
+    const long long pair = blockIdx.x;
+    if (pair >= batch) return;
+    const size_t mat = (size_t)kN * kN;
+    geneig::k1r_pair<kN>(S + mat * pair, W + (size_t)kN * pair, U + mat * pair, info + pair, W54(ws, pair, batch),
+                         reinterpret_cast<double*>(smem), vecs, vecs + RC54::VEC, (int)threadIdx.x);
+}
+
+__global__ void __launch_bounds__(64, GENEIG_K3R_WAVES_PER_EU) k3r_kernel(double* __restrict__ W,
+                                                                          geneig::cplx* __restrict__ U,
+                                                                          const int* __restrict__ info,
+                                                                          unsigned char* ws, int batch) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    __shared__ double ev[64];
+    __shared__ int perm[64];
+    const long long pair = blockIdx.x;
+    if (pair >= batch) return;
+    const size_t mat = (size_t)kN * kN;
+    geneig::k3r_pair<kN>(W + (size_t)kN * pair, U + mat * pair, info + pair, W54(ws, pair, batch),
+                         reinterpret_cast<double*>(smem), ev, perm, (int)threadIdx.x);
 }
 
 // hipFuncSetAttribute once per kernel per process, thread-safe (the reference
@@ -335,6 +374,56 @@ hipError_t run_sw(const SpinwaveModel& m, double* W, geneig::cplx* U, int* info,
     return hipSuccess;
 }
 
+// The real-arithmetic spin-wave solve: per part K1r, the FP64 QL stages, K3r.
+hipError_t pipeline_real(const geneig::cplx* s, double* W, geneig::cplx* u, int* info, int off, int P,
+                         unsigned char* w, hipStream_t st) {
+    const size_t mat = (size_t)kN * kN;
+    int* ovf = geneig_host::region_ctl(w, P, W54::PAIR_BYTES);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, ovf, 1);
+    RETURN_IF(hipGetLastError());
+    double* w_off = W + (size_t)kN * off;
+    geneig::cplx* u_off = u + mat * off;
+    hipLaunchKernelGGL(k1r_kernel, dim3((unsigned)P), dim3(64), RC54::LDS, st, s + mat * off, w_off, u_off,
+                       info + off, w, P);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2a_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), kK2aLds, st, info + off, w, P,
+                       geneig_host::ql_log_entries(), ovf);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2b_kernel, dim3((unsigned)P), dim3(64 * C54::R), 0, st, info + off, w, P);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k2x_kernel, dim3((unsigned)std::min(P, geneig::kListGridK2x)), dim3(64 * C54::R), 0, st,
+                       info + off, w, P, (const int*)ovf);
+    RETURN_IF(hipGetLastError());
+    hipLaunchKernelGGL(k3r_kernel, dim3((unsigned)P), dim3(64), RC54::LDS, st, w_off, u_off, info + off, w, P);
+    return hipGetLastError();
+}
+
+hipError_t run_real(const geneig::cplx* S, double* W, geneig::cplx* U, int* info, int batch, hipStream_t stream) {
+    if (batch <= 0) return hipSuccess;
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k1r_kernel), RC54::LDS));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k3r_kernel), RC54::LDS));
+    RETURN_IF(set_lds_once(reinterpret_cast<const void*>(k2a_kernel), kK2aLds));
+    const geneig_host::PlainLayout L = layout_of<geneig::MODE_FULL>(batch);
+    void* ws = nullptr;
+    RETURN_IF(geneig_host::workspace(stream, L.bytes, &ws));
+    unsigned char* base = static_cast<unsigned char*>(ws);
+    if (L.parts > 1) {
+        hipStream_t subs[geneig_host::kMaxParts];
+        RETURN_IF(geneig_host::fork_streams(stream, L.parts, subs));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < L.parts && e == hipSuccess; ++i) {
+            const int off = (int)((long long)batch * i / L.parts);
+            const int P = (int)((long long)batch * (i + 1) / L.parts) - off;
+            e = pipeline_real(S, W, U, info, off, P, base + L.region * i, subs[i]);
+        }
+        const hipError_t j = geneig_host::join_streams(stream, L.parts, subs);
+        return e != hipSuccess ? e : j;
+    }
+    for (int off = 0; off < batch; off += L.chunk)
+        RETURN_IF(pipeline_real(S, W, U, info, off, std::min(L.chunk, batch - off), base, stream));
+    return hipSuccess;
+}
+
 }  // namespace
 
 namespace geneig_host {
@@ -419,6 +508,12 @@ int geneig_spinwave_n54_launch(int nbonds, const int* d_shift, const int* d_r, c
     if (nbonds < 0 || (nbonds > 0 && (!d_shift || !d_r || !d_J)) || !d_q) return hipErrorInvalidValue;
     const SpinwaveModel m{nbonds, d_shift, d_r, d_J, d_q, d_diag};
     return run_sw(m, d_W, reinterpret_cast<geneig::cplx*>(d_U), d_info, batch_size, stream);
+}
+
+int geneig_eta_paired_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U,
+                                         int* d_info, int batch_size, hipStream_t stream) {
+    return run_real(reinterpret_cast<const geneig::cplx*>(d_S), d_W, reinterpret_cast<geneig::cplx*>(d_U), d_info,
+                    batch_size, stream);
 }
 
 int geneig_spinwave_n54_build(int nbonds, const int* d_shift, const int* d_r, const double* d_J, const double* d_q,

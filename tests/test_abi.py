@@ -220,6 +220,7 @@ def test_launchers_treat_empty_batches_as_no_ops(geneig, batch):
     fb = C.c_int(7)
     assert L.geneig_full_n54_batched_launch(None, None, None, None, None, batch, None) == 0
     assert L.geneig_full_n54_batched_inplace_launch(None, None, None, None, batch, None) == 0
+    assert L.geneig_eta_paired_n54_batched_launch(None, None, None, None, batch, None) == 0
     assert L.geneig_mixed_n54_batched_launch(None, None, None, None, None, batch, 0, C.c_double(0.0), None,
                                              C.byref(fb), None) == 0 and fb.value == 0
     fb.value = 7
