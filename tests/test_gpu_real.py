@@ -104,3 +104,58 @@ def test_real_form_small_batches(geneig, oracle, device, batch):
     Ah, Bh = to_host(A), to_host(B)
     Wo, Xo, io = oracle.zhegv(Ah, Bh)
     check_pairs(oracle, Ah, Bh, to_host(W), to_host(U), Wo, Xo, 54, "real form %d" % batch)
+
+
+def test_real_form_ql_log_overflow_path(geneig, oracle, device, monkeypatch):
+    """With the QL rotation log capped below one QL iteration
+    (GENEIG_QL_LOG_ENTRIES, read at every launch) every pair's QL runs in the
+    one-pair overflow kernel: the real form still meets the FP64 bar with the
+    pairing exact, and its eigenvalues agree with the logged path's to
+    rounding."""
+    A, B = geneig.gen_spinwave(27, 96, seed=5)
+    Ah, Bh = to_host(A), to_host(B)
+    Wo, Xo, _ = oracle.zhegv(Ah, Bh)
+    res = {}
+    for cap in (None, "16"):
+        if cap is not None:
+            monkeypatch.setenv("GENEIG_QL_LOG_ENTRIES", cap)
+        W, U, info = real_solve(geneig, B, device)
+        tag = "real form, log cap %s" % cap
+        assert int(info.abs().max()) == 0 and bool(paired_exact(W, U).all()), tag
+        check_pairs(oracle, Ah, Bh, to_host(W), to_host(U), Wo, Xo, 54, tag)
+        res[cap] = to_host(W)
+    assert np.abs(res[None] - res["16"]).max() <= 1e-13 * np.abs(res[None]).max()
+
+
+def test_real_form_workspace_chunking(geneig, oracle, device):
+    """A batch larger than one workspace chunk (GENEIG_WORKSPACE_MB=16: 141
+    pairs per chunk) goes through several K1r-K2-K3r rounds on the caller's
+    stream: the FP64 bar against the oracle and exact pairing on all 700
+    pairs."""
+    import os
+    import subprocess
+    import sys
+
+    from conftest import GOLDEN
+
+    code = r'''
+import os, sys, numpy as np, torch
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+import conftest, oracle as O
+g = conftest.load_package()
+A, B = g.gen_spinwave(27, 700, seed=12)
+W, U, info = g.empty_outputs(54, 700, "cuda")
+g.eta_paired_n54_batched(B, W, U, info); torch.cuda.synchronize()
+Wh, Uh = W.cpu().numpy(), U.cpu().numpy()
+part = np.concatenate([Uh[:, :, 27:], Uh[:, :, :27]], axis=2).conj()[:, ::-1, :]
+assert (Wh + Wh[:, ::-1] == 0).all() and (Uh == part).all()
+Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+Wo, Xo, io = O.zhegv(Ah, Bh)
+m = O.compare(Ah, Bh, Wh, Uh, Wo, Xo)
+assert int(info.abs().max()) == 0 and m["eig_rel_scaled"] < 1e-12 and m["phase_err"] < 1e-10, m
+print("ok", m["eig_rel_scaled"])
+''' % (os.path.dirname(GOLDEN), os.path.join(os.path.dirname(os.path.dirname(GOLDEN)), "oracle"))
+    env = dict(os.environ, GENEIG_WORKSPACE_MB="16")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    print(out.stdout, out.stderr[-2000:], flush=True)
+    assert out.returncode == 0 and "ok" in out.stdout
